@@ -1,0 +1,1017 @@
+/*
+ * C ABI of the gfx950 Maxwell-Bloch backend (declared in mbsolve_hip.h).
+ *
+ * Responsibilities: validate the problems, lay the runs out as one
+ * super-grid, upload constants / state / source tables / record
+ * descriptors, choose the tile geometry, and drive the time loop on one
+ * HIP stream (optionally replaying a captured hipGraph of launches).
+ */
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "mbh_tile.hpp"
+
+using namespace mbh;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+/* small device helpers */
+__global__ void set_u64(uint64_t* p, uint64_t v) { *p = v; }
+__global__ void add_u64(uint64_t* p, uint64_t v) { *p += v; }
+
+struct dev_buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct run_layout {
+    uint64_t base;      /* super-grid cell of cell 0 */
+    uint64_t cells;     /* N_x */
+    uint64_t scratch;   /* doubles */
+    uint64_t scratch_off; /* doubles, into the shared scratch allocation */
+    uint32_t dstride_ref; /* reference density stride */
+};
+
+} /* namespace */
+
+struct mbh_ctx {
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+
+    uint32_t algo = 0, levels = 0, form = 0;
+    uint64_t n_t = 0;
+    uint32_t DC = 0;        /* stored density components per cell */
+    uint32_t dref = 0;      /* reference density components per cell */
+    bool fp32 = false;
+    bool zform = false;
+
+    uint64_t G = 0, dstride = 0;
+    std::vector<run_layout> runs;
+
+    /* tiling */
+    uint32_t C = 1, halo = 1, owned = 62, tb = 1;
+    uint64_t num_tiles = 0;
+    tile_launch_fn launch = nullptr;
+    zconv_fn to_y = nullptr, from_y = nullptr;
+
+    /* device memory */
+    std::vector<void*> allocs;
+    void* state[2][4] = { { nullptr } }; /* [buffer][e,h,p,d] */
+    int cur = 0;
+    int32_t* mat = nullptr;
+    uint8_t* hzero = nullptr;
+    double* fdtd = nullptr;
+    double* qm = nullptr;
+    uint64_t qstride = 0;
+    dev_source* src = nullptr;
+    uint32_t* tile_src = nullptr;
+    dev_record* rec = nullptr;
+    uint32_t* tile_rec_ptr = nullptr;
+    uint32_t* tile_rec_idx = nullptr;
+    double* src_data = nullptr;
+    double* scratch = nullptr;
+    uint64_t* n_base = nullptr;
+
+    /* graphs */
+    uint32_t graph_launches = 1;
+    hipGraphExec_t graph[2] = { nullptr, nullptr }; /* per start buffer */
+
+    uint64_t last_launches = 0;
+    uint64_t state_bytes = 0;
+
+    ~mbh_ctx()
+    {
+        if (device >= 0) {
+            (void)hipSetDevice(device);
+        }
+        for (auto& g : graph) {
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+            }
+        }
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+        }
+        for (void* p : allocs) {
+            (void)hipFree(p);
+        }
+        if (ev0) {
+            (void)hipEventDestroy(ev0);
+        }
+        if (ev1) {
+            (void)hipEventDestroy(ev1);
+        }
+        if (stream) {
+            (void)hipStreamDestroy(stream);
+        }
+    }
+};
+
+namespace {
+
+int fail(mbh_ctx* ctx, int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    if (ctx) {
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIP_TRY(ctx, call)                                                    \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            return fail(ctx, (e_ == hipErrorOutOfMemory) ? MBH_ERR_NOMEM       \
+                                                         : MBH_ERR_HIP,       \
+                        "%s failed: %s", #call, hipGetErrorString(e_));       \
+        }                                                                     \
+    } while (0)
+
+int dalloc(mbh_ctx* ctx, void** p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) {
+        bytes = 8;
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(ctx, (e == hipErrorOutOfMemory) ? MBH_ERR_NOMEM
+                                                    : MBH_ERR_HIP,
+                    "hipMalloc(%zu bytes) failed: %s", bytes,
+                    hipGetErrorString(e));
+    }
+    ctx->allocs.push_back(*p);
+    return MBH_OK;
+}
+
+template<typename T>
+int upload(mbh_ctx* ctx, T** dst, const T* src, size_t count)
+{
+    int rc = dalloc(ctx, (void**)dst, count * sizeof(T));
+    if (rc) {
+        return rc;
+    }
+    if (count) {
+        HIP_TRY(ctx, hipMemcpy(*dst, src, count * sizeof(T),
+                               hipMemcpyHostToDevice));
+    }
+    return MBH_OK;
+}
+
+uint32_t ref_density_stride(uint32_t algo, uint32_t N)
+{
+    switch (algo) {
+    case MBH_ALGO_REG_CAYLEY:
+        return MBH_DENSITY_STRIDE_CAYLEY(N);
+    case MBH_ALGO_CVR_RODR:
+        return MBH_DENSITY_STRIDE_RODR(N);
+    default:
+        return 0;
+    }
+}
+
+uint64_t qm_stride_of(uint32_t algo, uint32_t N)
+{
+    switch (algo) {
+    case MBH_ALGO_REG_CAYLEY:
+        return MBH_CAYLEY_STRIDE(N);
+    case MBH_ALGO_CVR_RODR:
+        return MBH_RODR_STRIDE(N);
+    default:
+        return 0;
+    }
+}
+
+int validate(const mbh_problem* p, uint32_t r, const mbh_problem* p0,
+             uint32_t form)
+{
+    if (p->abi_version != MBH_ABI_VERSION) {
+        return fail(nullptr, MBH_ERR_INVALID,
+                    "run %u: abi_version %u, library expects %u", r,
+                    p->abi_version, MBH_ABI_VERSION);
+    }
+    if (p->algo > MBH_ALGO_CVR_RODR) {
+        return fail(nullptr, MBH_ERR_INVALID, "run %u: unknown algo %u", r,
+                    p->algo);
+    }
+    if (p->algo != MBH_ALGO_NOOP &&
+        (p->num_levels < 2 || p->num_levels > MBH_MAX_LEVELS)) {
+        return fail(nullptr, MBH_ERR_INVALID,
+                    "run %u: num_levels %u outside [2, %u]", r, p->num_levels,
+                    MBH_MAX_LEVELS);
+    }
+    if (p->algo != p0->algo || p->num_levels != p0->num_levels ||
+        p->num_timesteps != p0->num_timesteps) {
+        return fail(nullptr, MBH_ERR_INVALID,
+                    "run %u: algo, num_levels and num_timesteps must be equal "
+                    "for all runs of one context",
+                    r);
+    }
+    if (p->num_gridpoints < 1 || p->num_gridpoints > (1ull << 40)) {
+        return fail(nullptr, MBH_ERR_INVALID, "run %u: bad num_gridpoints",
+                    r);
+    }
+    if (p->num_materials < 1 || !p->fdtd_consts || !p->mat_idx ||
+        !p->e_init || !p->h_init) {
+        return fail(nullptr, MBH_ERR_INVALID,
+                    "run %u: missing materials or initial state", r);
+    }
+    const uint64_t qs = qm_stride_of(p->algo, p->num_levels);
+    if (p->algo != MBH_ALGO_NOOP) {
+        if (p->qm_stride != qs || !p->qm_consts || !p->d_init) {
+            return fail(nullptr, MBH_ERR_INVALID,
+                        "run %u: qm_stride %llu (expected %llu) or missing "
+                        "qm_consts/d_init",
+                        r, (unsigned long long)p->qm_stride,
+                        (unsigned long long)qs);
+        }
+    }
+    if (p->algo == MBH_ALGO_CVR_RODR &&
+        (form == MBH_FORM_MIXED || form == MBH_FORM_ZFORM64)) {
+        if (!p->zform_consts ||
+            p->zform_stride != MBH_ZFORM_STRIDE(p->num_levels)) {
+            return fail(nullptr, MBH_ERR_INVALID,
+                        "run %u: the rotated-basis form needs zform_consts "
+                        "with stride %u",
+                        r, MBH_ZFORM_STRIDE(p->num_levels));
+        }
+    }
+    for (uint64_t i = 0; i < p->num_gridpoints; i++) {
+        if (p->mat_idx[i] >= p->num_materials) {
+            return fail(nullptr, MBH_ERR_INVALID,
+                        "run %u: mat_idx[%llu] = %u >= num_materials", r,
+                        (unsigned long long)i, p->mat_idx[i]);
+        }
+    }
+    if (p->num_sources > MBH_MAX_SOURCES ||
+        (p->num_sources && (!p->sources || !p->source_data))) {
+        return fail(nullptr, MBH_ERR_INVALID, "run %u: bad source list", r);
+    }
+    for (uint32_t k = 0; k < p->num_sources; k++) {
+        if (p->sources[k].x_idx >= p->num_gridpoints ||
+            p->sources[k].type > MBH_SRC_SOFT) {
+            return fail(nullptr, MBH_ERR_INVALID,
+                        "run %u: source %u at x_idx %u outside the grid or of "
+                        "unknown type",
+                        r, k, p->sources[k].x_idx);
+        }
+    }
+    if (p->num_records > MBH_MAX_RECORDS ||
+        (p->num_records && !p->records)) {
+        return fail(nullptr, MBH_ERR_INVALID, "run %u: bad record list", r);
+    }
+    for (uint32_t k = 0; k < p->num_records; k++) {
+        const mbh_record& R = p->records[k];
+        if (R.type > MBH_REC_INVERSION || R.cols < 1 ||
+            R.position_idx + R.cols > p->num_gridpoints ||
+            R.offset_real + R.rows * R.cols > p->scratch_size ||
+            !(R.interval_ratio > 0.0) || R.interval_ratio > 1.0) {
+            return fail(nullptr, MBH_ERR_INVALID,
+                        "run %u: record %u is inconsistent (type %u, position "
+                        "%llu, cols %llu, rows %llu)",
+                        r, k, R.type, (unsigned long long)R.position_idx,
+                        (unsigned long long)R.cols,
+                        (unsigned long long)R.rows);
+        }
+        if (R.type == MBH_REC_DENSITY && p->algo != MBH_ALGO_NOOP &&
+            (R.row_idx >= p->num_levels || R.col_idx >= p->num_levels)) {
+            return fail(nullptr, MBH_ERR_INVALID,
+                        "run %u: density record %u index out of range", r, k);
+        }
+    }
+    return MBH_OK;
+}
+
+const kernel_entry* find_kernel(uint32_t algo, uint32_t levels, uint32_t form,
+                                uint32_t C)
+{
+    const kernel_entry* tabs[3] = { k_table_basic, k_table_rodr,
+                                    k_table_zform };
+    const uint32_t ns[3] = { k_table_basic_n, k_table_rodr_n,
+                             k_table_zform_n };
+    for (int t = 0; t < 3; t++) {
+        for (uint32_t i = 0; i < ns[t]; i++) {
+            const kernel_entry& k = tabs[t][i];
+            if (k.algo == algo && k.levels == levels && k.form == form &&
+                k.cells_per_lane == C) {
+                return &k;
+            }
+        }
+    }
+    return nullptr;
+}
+
+/* Kernel form implementing the requested numerical form. */
+uint32_t kernel_form(uint32_t algo, uint32_t form)
+{
+    if (algo != MBH_ALGO_CVR_RODR && form == MBH_FORM_ZFORM64) {
+        return MBH_FORM_FP64;
+    }
+    return form;
+}
+
+/* Tile geometry: C cells per lane, halo = temporal block.  Prefers the
+ * largest temporal block whose halo overhead stays below ~25% while
+ * keeping at least ~4 wavefronts per CU busy. */
+void choose_geometry(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
+{
+    const uint32_t algo = ctx->algo, levels = ctx->levels;
+    const uint32_t kform = kernel_form(algo, ctx->form);
+    std::vector<uint32_t> cs;
+    for (uint32_t C : { 4u, 2u, 1u }) {
+        if (find_kernel(algo, levels, kform, C)) {
+            cs.push_back(C);
+        }
+    }
+    if (req_tile) {
+        uint32_t C = std::max(1u, req_tile / 64u);
+        bool ok = std::find(cs.begin(), cs.end(), C) != cs.end();
+        cs.assign(1, ok ? C : cs.back());
+    }
+    const double target_tiles = 1024.0;
+    double best = -1.0;
+    uint32_t bc = cs.back(), btb = 1;
+    for (uint32_t C : cs) {
+        const uint32_t span = 64u * C;
+        for (uint32_t tb : { 64u, 48u, 32u, 24u, 16u, 12u, 8u, 6u, 4u, 2u,
+                             1u }) {
+            if (req_tb && tb != req_tb) {
+                continue;
+            }
+            if (2u * tb >= span) {
+                continue;
+            }
+            const uint32_t own = span - 2u * tb;
+            const double tiles = std::ceil((double)ctx->G / own);
+            const double eff = (double)own / span;
+            /* fraction of the machine kept busy */
+            const double fill = std::min(1.0, tiles / target_tiles);
+            /* launch amortisation: a launch costs roughly as much as a few
+             * microseconds of work */
+            const double amort = tb / (tb + 2.0);
+            const double score = eff * fill * amort;
+            if (score > best) {
+                best = score;
+                bc = C;
+                btb = tb;
+            }
+        }
+    }
+    if (req_tb && best < 0.0) {
+        /* requested block too large for every tile: use the widest tile */
+        bc = cs.front();
+        btb = std::min(req_tb, 32u * bc - 1u);
+    }
+    ctx->C = bc;
+    ctx->tb = btb;
+    ctx->halo = btb;
+    ctx->owned = 64u * bc - 2u * btb;
+    ctx->num_tiles = (ctx->G + ctx->owned - 1) / ctx->owned;
+}
+
+KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
+                uint32_t nsteps)
+{
+    KArgs k{};
+    const int ob = 1 - in_buf;
+    k.e_in = ctx->state[in_buf][0];
+    k.h_in = ctx->state[in_buf][1];
+    k.p_in = ctx->state[in_buf][2];
+    k.d_in = ctx->state[in_buf][3];
+    k.e_out = ctx->state[ob][0];
+    k.h_out = ctx->state[ob][1];
+    k.p_out = ctx->state[ob][2];
+    k.d_out = ctx->state[ob][3];
+    k.G = ctx->G;
+    k.dstride = ctx->dstride;
+    k.mat = ctx->mat;
+    k.hzero = ctx->hzero;
+    k.fdtd = ctx->fdtd;
+    k.qm = ctx->qm;
+    k.qstride = ctx->qstride;
+    k.src = ctx->src;
+    k.tile_src = ctx->tile_src;
+    k.rec = ctx->rec;
+    k.tile_rec_ptr = ctx->tile_rec_ptr;
+    k.tile_rec_idx = ctx->tile_rec_idx;
+    k.n_base = ctx->n_base;
+    k.n_off = n_off;
+    k.nsteps = nsteps;
+    k.halo = ctx->halo;
+    k.owned = ctx->owned;
+    return k;
+}
+
+int build_graph(mbh_ctx* ctx, int start_buf)
+{
+    hipGraph_t g = nullptr;
+    HIP_TRY(ctx, hipStreamBeginCapture(ctx->stream,
+                                       hipStreamCaptureModeThreadLocal));
+    int buf = start_buf;
+    hipError_t le = hipSuccess;
+    for (uint32_t j = 0; j < ctx->graph_launches && le == hipSuccess; j++) {
+        KArgs k = make_args(ctx, buf, (uint64_t)j * ctx->tb, ctx->tb);
+        le = ctx->launch(k, ctx->num_tiles, ctx->stream);
+        buf = 1 - buf;
+    }
+    if (le == hipSuccess) {
+        hipLaunchKernelGGL(add_u64, dim3(1), dim3(1), 0, ctx->stream,
+                           ctx->n_base,
+                           (uint64_t)ctx->graph_launches * ctx->tb);
+        le = hipGetLastError();
+    }
+    hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+        if (g) {
+            (void)hipGraphDestroy(g);
+        }
+        return fail(ctx, MBH_ERR_HIP, "graph capture failed: %s",
+                    hipGetErrorString(le != hipSuccess ? le : ce));
+    }
+    hipError_t ie = hipGraphInstantiate(&ctx->graph[start_buf], g, nullptr,
+                                        nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        ctx->graph[start_buf] = nullptr;
+        return fail(ctx, MBH_ERR_HIP, "graph instantiation failed: %s",
+                    hipGetErrorString(ie));
+    }
+    return MBH_OK;
+}
+
+} /* namespace */
+
+extern "C" {
+
+const char* mbh_version(void) { return "mbsolve-hip 1 gfx950"; }
+
+int mbh_device_count(int* count)
+{
+    if (!count) {
+        return fail(nullptr, MBH_ERR_INVALID, "count is NULL");
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return MBH_OK;
+    }
+    *count = n;
+    return MBH_OK;
+}
+
+void mbh_default_options(mbh_options* opt)
+{
+    if (!opt) {
+        return;
+    }
+    std::memset(opt, 0, sizeof(*opt));
+    opt->form = MBH_FORM_FP64;
+    opt->device = -1;
+}
+
+int mbh_create(const mbh_problem* runs, uint32_t num_runs,
+               const mbh_options* opt_in, mbh_ctx** out)
+{
+    if (!out) {
+        return fail(nullptr, MBH_ERR_INVALID, "out is NULL");
+    }
+    *out = nullptr;
+    if (!runs || num_runs == 0) {
+        return fail(nullptr, MBH_ERR_INVALID, "no runs given");
+    }
+    mbh_options opt;
+    mbh_default_options(&opt);
+    if (opt_in) {
+        opt = *opt_in;
+    }
+    if (opt.form > MBH_FORM_ZFORM64) {
+        return fail(nullptr, MBH_ERR_INVALID, "unknown form %u", opt.form);
+    }
+    for (uint32_t r = 0; r < num_runs; r++) {
+        int rc = validate(&runs[r], r, &runs[0], opt.form);
+        if (rc) {
+            return rc;
+        }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        return fail(nullptr, MBH_ERR_NODEVICE, "no HIP device available");
+    }
+
+    std::unique_ptr<mbh_ctx> ctx(new mbh_ctx());
+    mbh_ctx* c = ctx.get();
+    if (opt.device >= 0) {
+        if (opt.device >= ndev) {
+            return fail(nullptr, MBH_ERR_INVALID, "device %d of %d",
+                        opt.device, ndev);
+        }
+        c->device = opt.device;
+        HIP_TRY(nullptr, hipSetDevice(c->device));
+    } else {
+        HIP_TRY(nullptr, hipGetDevice(&c->device));
+    }
+
+    const mbh_problem& p0 = runs[0];
+    c->algo = p0.algo;
+    c->levels = (p0.algo == MBH_ALGO_NOOP) ? 0 : p0.num_levels;
+    c->form = opt.form;
+    c->n_t = p0.num_timesteps;
+    c->dref = ref_density_stride(c->algo, c->levels);
+    c->DC = c->dref;
+    c->fp32 = (opt.form == MBH_FORM_MIXED);
+    c->zform = (c->algo == MBH_ALGO_CVR_RODR) &&
+        (opt.form == MBH_FORM_MIXED || opt.form == MBH_FORM_ZFORM64);
+    const size_t esz = c->fp32 ? sizeof(float) : sizeof(double);
+
+    /* super-grid layout */
+    uint64_t G = 0, scratch_total = 0;
+    uint32_t nmat_total = 0;
+    for (uint32_t r = 0; r < num_runs; r++) {
+        run_layout L;
+        L.base = G;
+        L.cells = runs[r].num_gridpoints;
+        L.scratch = runs[r].scratch_size;
+        L.scratch_off = scratch_total;
+        L.dstride_ref = c->dref;
+        c->runs.push_back(L);
+        G += L.cells + 1;
+        scratch_total += runs[r].scratch_size;
+        nmat_total += runs[r].num_materials;
+    }
+    c->G = G;
+    c->dstride = (G + 63) & ~uint64_t(63);
+
+    /* per-cell material index and H boundary flags */
+    std::vector<int32_t> mat(G, -1);
+    std::vector<uint8_t> hz(G, 1);
+    std::vector<double> fdtd((size_t)nmat_total * FDTD_STRIDE);
+    c->qstride = c->zform ? runs[0].zform_stride : runs[0].qm_stride;
+    std::vector<double> qm((size_t)nmat_total * c->qstride);
+    uint32_t moff = 0;
+    for (uint32_t r = 0; r < num_runs; r++) {
+        const mbh_problem& p = runs[r];
+        const run_layout& L = c->runs[r];
+        for (uint64_t i = 0; i < L.cells; i++) {
+            mat[L.base + i] = (int32_t)(moff + p.mat_idx[i]);
+            hz[L.base + i] = (i == 0) ? 1 : 0;
+        }
+        for (uint32_t m = 0; m < p.num_materials; m++) {
+            for (int j = 0; j < 4; j++) {
+                fdtd[(size_t)(moff + m) * FDTD_STRIDE + j] =
+                    p.fdtd_consts[(size_t)m * 4 + j];
+            }
+            fdtd[(size_t)(moff + m) * FDTD_STRIDE + 4] = p.dx_inv;
+            const double* q = c->zform ? p.zform_consts : p.qm_consts;
+            if (c->qstride) {
+                std::memcpy(&qm[(size_t)(moff + m) * c->qstride],
+                            q + (size_t)m * c->qstride,
+                            c->qstride * sizeof(double));
+            }
+        }
+        moff += p.num_materials;
+    }
+
+    /* kernel and geometry */
+    /* graphs hold an even number of launches so that the ping-pong buffer
+     * parity is the same before and after a replay */
+    c->graph_launches =
+        (opt.graph_launches > 1) ? ((opt.graph_launches + 1u) & ~1u) : 1u;
+    choose_geometry(c, opt.temporal_block, opt.tile_cells);
+    const kernel_entry* ke =
+        find_kernel(c->algo, c->levels, kernel_form(c->algo, c->form), c->C);
+    if (!ke) {
+        return fail(nullptr, MBH_ERR_UNSUPPORTED,
+                    "no kernel for algo %u, %u levels, form %u", c->algo,
+                    c->levels, c->form);
+    }
+    c->launch = ke->fn;
+    if (c->zform) {
+        for (uint32_t i = 0; i < z_table_n; i++) {
+            if (z_table[i].levels == c->levels &&
+                z_table[i].fp32 == (c->fp32 ? 1u : 0u)) {
+                c->to_y = z_table[i].to_y;
+                c->from_y = z_table[i].from_y;
+            }
+        }
+        if (!c->to_y) {
+            return fail(nullptr, MBH_ERR_UNSUPPORTED,
+                        "no z-form conversion for %u levels", c->levels);
+        }
+    }
+
+    HIP_TRY(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIP_TRY(c, hipEventCreate(&c->ev0));
+    HIP_TRY(c, hipEventCreate(&c->ev1));
+
+    int rc;
+#define RC(x)                                                                 \
+    do {                                                                      \
+        if ((rc = (x)) != MBH_OK) {                                           \
+            g_last_error = c->err;                                            \
+            return rc;                                                        \
+        }                                                                     \
+    } while (0)
+
+    RC(upload(c, &c->mat, mat.data(), mat.size()));
+    RC(upload(c, &c->hzero, hz.data(), hz.size()));
+    RC(upload(c, &c->fdtd, fdtd.data(), fdtd.size()));
+    RC(upload(c, &c->qm, qm.data(), qm.size()));
+
+    /* state buffers: e, h, p [G], density [DC][dstride] */
+    for (int b = 0; b < 2; b++) {
+        RC(dalloc(c, &c->state[b][0], G * esz));
+        RC(dalloc(c, &c->state[b][1], G * esz));
+        RC(dalloc(c, &c->state[b][2], G * esz));
+        RC(dalloc(c, &c->state[b][3],
+                  std::max<uint64_t>(1, c->DC) * c->dstride * esz));
+        for (int j = 0; j < 4; j++) {
+            HIP_TRY(c, hipMemset(c->state[b][j], 0,
+                                 (j < 3 ? G : std::max<uint64_t>(1, c->DC) *
+                                          c->dstride) *
+                                     esz));
+        }
+    }
+    c->state_bytes = (3 * G + (uint64_t)c->DC * c->dstride) * esz;
+
+    /* initial state */
+    {
+        std::vector<double> e(G, 0.0), h(G, 0.0);
+        std::vector<double> d((size_t)c->dref * c->dstride, 0.0);
+        for (uint32_t r = 0; r < num_runs; r++) {
+            const mbh_problem& p = runs[r];
+            const run_layout& L = c->runs[r];
+            for (uint64_t i = 0; i < L.cells; i++) {
+                e[L.base + i] = p.e_init[i];
+                h[L.base + i] = p.h_init[i];
+                for (uint32_t j = 0; j < c->dref; j++) {
+                    d[(size_t)j * c->dstride + L.base + i] =
+                        p.d_init[i * c->dref + j];
+                }
+            }
+            h[L.base + L.cells] = p.h_init[L.cells];
+        }
+        if (c->fp32) {
+            std::vector<float> ef(e.begin(), e.end()), hf(h.begin(), h.end());
+            HIP_TRY(c, hipMemcpy(c->state[0][0], ef.data(), G * esz,
+                                 hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMemcpy(c->state[0][1], hf.data(), G * esz,
+                                 hipMemcpyHostToDevice));
+        } else {
+            HIP_TRY(c, hipMemcpy(c->state[0][0], e.data(), G * esz,
+                                 hipMemcpyHostToDevice));
+            HIP_TRY(c, hipMemcpy(c->state[0][1], h.data(), G * esz,
+                                 hipMemcpyHostToDevice));
+        }
+        if (c->dref) {
+            if (c->zform) {
+                double* dtmp = nullptr;
+                RC(upload(c, &dtmp, d.data(), d.size()));
+                ZArgs za{ dtmp, c->state[0][3], G, c->dstride, c->mat, c->qm,
+                          c->qstride };
+                HIP_TRY(c, c->to_y(za, c->stream));
+                HIP_TRY(c, hipStreamSynchronize(c->stream));
+            } else if (c->fp32) {
+                std::vector<float> df(d.begin(), d.end());
+                HIP_TRY(c, hipMemcpy(c->state[0][3], df.data(),
+                                     df.size() * sizeof(float),
+                                     hipMemcpyHostToDevice));
+            } else {
+                HIP_TRY(c, hipMemcpy(c->state[0][3], d.data(),
+                                     d.size() * sizeof(double),
+                                     hipMemcpyHostToDevice));
+            }
+        }
+        c->cur = 0;
+    }
+
+    /* source tables */
+    {
+        uint64_t total = 0;
+        for (uint32_t r = 0; r < num_runs; r++) {
+            total += (uint64_t)runs[r].num_sources * c->n_t;
+        }
+        RC(dalloc(c, (void**)&c->src_data, total * sizeof(double)));
+        std::vector<dev_source> srcs;
+        uint64_t off = 0;
+        for (uint32_t r = 0; r < num_runs; r++) {
+            const mbh_problem& p = runs[r];
+            for (uint32_t k = 0; k < p.num_sources; k++) {
+                HIP_TRY(c, hipMemcpy(c->src_data + off,
+                                     p.source_data + (size_t)k * c->n_t,
+                                     c->n_t * sizeof(double),
+                                     hipMemcpyHostToDevice));
+                dev_source s{};
+                s.gx = c->runs[r].base + p.sources[k].x_idx;
+                s.data = c->src_data + off;
+                s.type = p.sources[k].type;
+                srcs.push_back(s);
+                off += c->n_t;
+            }
+        }
+        /* stable: sources at one cell keep the reference's list order */
+        std::stable_sort(srcs.begin(), srcs.end(),
+                         [](const dev_source& a, const dev_source& b) {
+                             return a.gx < b.gx;
+                         });
+        std::vector<uint32_t> ts(2 * c->num_tiles);
+        for (uint64_t t = 0; t < c->num_tiles; t++) {
+            const int64_t lo = (int64_t)(t * c->owned) - c->halo;
+            const int64_t hi = lo + 64 * (int64_t)c->C;
+            auto a = std::lower_bound(srcs.begin(), srcs.end(), lo,
+                                      [](const dev_source& s, int64_t v) {
+                                          return (int64_t)s.gx < v;
+                                      });
+            auto b = std::lower_bound(srcs.begin(), srcs.end(), hi,
+                                      [](const dev_source& s, int64_t v) {
+                                          return (int64_t)s.gx < v;
+                                      });
+            ts[2 * t] = (uint32_t)(a - srcs.begin());
+            ts[2 * t + 1] = (uint32_t)(b - srcs.begin());
+        }
+        RC(upload(c, &c->src, srcs.data(), srcs.size()));
+        RC(upload(c, &c->tile_src, ts.data(), ts.size()));
+    }
+
+    /* record scratch and descriptors */
+    {
+        RC(dalloc(c, (void**)&c->scratch,
+                  std::max<uint64_t>(1, scratch_total) * sizeof(double)));
+        HIP_TRY(c, hipMemset(c->scratch, 0,
+                             std::max<uint64_t>(1, scratch_total) *
+                                 sizeof(double)));
+        std::vector<dev_record> recs;
+        std::vector<std::pair<uint64_t, uint32_t>> tile_rec;
+        for (uint32_t r = 0; r < num_runs; r++) {
+            const mbh_problem& p = runs[r];
+            const run_layout& L = c->runs[r];
+            for (uint32_t k = 0; k < p.num_records; k++) {
+                const mbh_record& R = p.records[k];
+                dev_record d{};
+                d.gbegin = L.base + R.position_idx;
+                d.cols = R.cols;
+                d.rows = R.rows;
+                d.ratio = R.interval_ratio;
+                d.dst = c->scratch + L.scratch_off + R.offset_real;
+                d.type = R.type;
+                d.row_idx = R.row_idx;
+                d.col_idx = R.col_idx;
+                const uint32_t id = (uint32_t)recs.size();
+                recs.push_back(d);
+                const uint64_t t0 = d.gbegin / c->owned;
+                const uint64_t t1 = (d.gbegin + d.cols - 1) / c->owned;
+                for (uint64_t t = t0; t <= t1; t++) {
+                    tile_rec.emplace_back(t, id);
+                }
+            }
+        }
+        std::stable_sort(tile_rec.begin(), tile_rec.end(),
+                         [](const std::pair<uint64_t, uint32_t>& a,
+                            const std::pair<uint64_t, uint32_t>& b) {
+                             return a.first < b.first;
+                         });
+        std::vector<uint32_t> ptr(c->num_tiles + 1, 0), idx;
+        idx.reserve(tile_rec.size());
+        for (auto& tr : tile_rec) {
+            ptr[tr.first + 1]++;
+            idx.push_back(tr.second);
+        }
+        for (uint64_t t = 0; t < c->num_tiles; t++) {
+            ptr[t + 1] += ptr[t];
+        }
+        RC(upload(c, &c->rec, recs.data(), recs.size()));
+        RC(upload(c, &c->tile_rec_ptr, ptr.data(), ptr.size()));
+        RC(upload(c, &c->tile_rec_idx, idx.data(), idx.size()));
+    }
+
+    RC(dalloc(c, (void**)&c->n_base, sizeof(uint64_t)));
+    HIP_TRY(c, hipMemset(c->n_base, 0, sizeof(uint64_t)));
+    HIP_TRY(c, hipDeviceSynchronize());
+#undef RC
+    *out = ctx.release();
+    return MBH_OK;
+}
+
+int mbh_run(mbh_ctx* ctx, uint64_t n_begin, uint64_t n_end)
+{
+    if (!ctx) {
+        return fail(nullptr, MBH_ERR_INVALID, "ctx is NULL");
+    }
+    if (n_begin > n_end || n_end > ctx->n_t) {
+        return fail(ctx, MBH_ERR_INVALID,
+                    "step range [%llu, %llu) outside [0, %llu)",
+                    (unsigned long long)n_begin, (unsigned long long)n_end,
+                    (unsigned long long)ctx->n_t);
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL(set_u64, dim3(1), dim3(1), 0, ctx->stream,
+                       ctx->n_base, n_begin);
+    HIP_TRY(ctx, hipGetLastError());
+
+    uint64_t base = n_begin; /* value *n_base will hold at this point */
+    uint64_t n = n_begin;
+    uint64_t launches = 0;
+    const uint64_t per_graph = (uint64_t)ctx->graph_launches * ctx->tb;
+    while (n < n_end) {
+        const uint64_t left = n_end - n;
+        if (ctx->graph_launches > 1 && left >= per_graph) {
+            if (!ctx->graph[ctx->cur]) {
+                int rc = build_graph(ctx, ctx->cur);
+                if (rc) {
+                    return rc;
+                }
+            }
+            HIP_TRY(ctx, hipGraphLaunch(ctx->graph[ctx->cur], ctx->stream));
+            /* graph_launches is even: the buffer parity is unchanged */
+            n += per_graph;
+            base += per_graph;
+            launches += ctx->graph_launches;
+            continue;
+        }
+        const uint32_t steps = (uint32_t)std::min<uint64_t>(ctx->tb, left);
+        KArgs k = make_args(ctx, ctx->cur, n - base, steps);
+        HIP_TRY(ctx, ctx->launch(k, ctx->num_tiles, ctx->stream));
+        ctx->cur = 1 - ctx->cur;
+        n += steps;
+        launches++;
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timed = true;
+    ctx->last_launches = launches;
+    return MBH_OK;
+}
+
+int mbh_synchronize(mbh_ctx* ctx)
+{
+    if (!ctx) {
+        return fail(nullptr, MBH_ERR_INVALID, "ctx is NULL");
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return MBH_OK;
+}
+
+int mbh_elapsed_ms(mbh_ctx* ctx, double* ms)
+{
+    if (!ctx || !ms) {
+        return fail(ctx, MBH_ERR_INVALID, "ctx or ms is NULL");
+    }
+    if (!ctx->timed) {
+        return fail(ctx, MBH_ERR_INVALID, "no mbh_run recorded yet");
+    }
+    HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+    float f = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&f, ctx->ev0, ctx->ev1));
+    *ms = (double)f;
+    return MBH_OK;
+}
+
+int mbh_fetch_records(mbh_ctx* ctx, uint32_t run, double* dst,
+                      uint64_t count)
+{
+    if (!ctx || !dst) {
+        return fail(ctx, MBH_ERR_INVALID, "ctx or dst is NULL");
+    }
+    if (run >= ctx->runs.size() || count > ctx->runs[run].scratch) {
+        return fail(ctx, MBH_ERR_INVALID,
+                    "run %u / count %llu out of range", run,
+                    (unsigned long long)count);
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipMemcpy(dst, ctx->scratch + ctx->runs[run].scratch_off,
+                           count * sizeof(double), hipMemcpyDeviceToHost));
+    return MBH_OK;
+}
+
+int mbh_fetch_state(mbh_ctx* ctx, uint32_t run, double* e, double* h,
+                    double* p, double* d)
+{
+    if (!ctx) {
+        return fail(nullptr, MBH_ERR_INVALID, "ctx is NULL");
+    }
+    if (run >= ctx->runs.size()) {
+        return fail(ctx, MBH_ERR_INVALID, "run %u out of range", run);
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    const run_layout& L = ctx->runs[run];
+    const size_t esz = ctx->fp32 ? sizeof(float) : sizeof(double);
+    auto get = [&](const void* src, uint64_t first, uint64_t count,
+                   double* out) -> int {
+        if (ctx->fp32) {
+            std::vector<float> tmp(count);
+            HIP_TRY(ctx, hipMemcpy(tmp.data(),
+                                   (const char*)src + first * esz,
+                                   count * esz, hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < count; i++) {
+                out[i] = tmp[i];
+            }
+        } else {
+            HIP_TRY(ctx, hipMemcpy(out, (const char*)src + first * esz,
+                                   count * esz, hipMemcpyDeviceToHost));
+        }
+        return MBH_OK;
+    };
+    int rc;
+    void* const* S = ctx->state[ctx->cur];
+    if (e && (rc = get(S[0], L.base, L.cells, e))) {
+        return rc;
+    }
+    if (h && (rc = get(S[1], L.base, L.cells + 1, h))) {
+        return rc;
+    }
+    if (p && (rc = get(S[2], L.base, L.cells, p))) {
+        return rc;
+    }
+    if (d && ctx->dref) {
+        std::vector<double> comp(L.cells);
+        if (ctx->zform) {
+            double* dtmp = nullptr;
+            rc = dalloc(ctx, (void**)&dtmp,
+                        (size_t)ctx->dref * ctx->dstride * sizeof(double));
+            if (rc) {
+                return rc;
+            }
+            ZArgs za{ dtmp, S[3], ctx->G, ctx->dstride, ctx->mat, ctx->qm,
+                      ctx->qstride };
+            HIP_TRY(ctx, ctx->from_y(za, ctx->stream));
+            HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+            for (uint32_t j = 0; j < ctx->dref; j++) {
+                HIP_TRY(ctx, hipMemcpy(comp.data(),
+                                       dtmp + (size_t)j * ctx->dstride +
+                                           L.base,
+                                       L.cells * sizeof(double),
+                                       hipMemcpyDeviceToHost));
+                for (uint64_t i = 0; i < L.cells; i++) {
+                    d[i * ctx->dref + j] = comp[i];
+                }
+            }
+            ctx->allocs.pop_back();
+            (void)hipFree(dtmp);
+        } else {
+            for (uint32_t j = 0; j < ctx->dref; j++) {
+                if ((rc = get(S[3], (uint64_t)j * ctx->dstride + L.base,
+                              L.cells, comp.data()))) {
+                    return rc;
+                }
+                for (uint64_t i = 0; i < L.cells; i++) {
+                    d[i * ctx->dref + j] = comp[i];
+                }
+            }
+        }
+    }
+    return MBH_OK;
+}
+
+int mbh_get_info(const mbh_ctx* ctx, mbh_info* info)
+{
+    if (!ctx || !info) {
+        return fail(nullptr, MBH_ERR_INVALID, "ctx or info is NULL");
+    }
+    std::memset(info, 0, sizeof(*info));
+    info->form = ctx->form;
+    info->temporal_block = ctx->tb;
+    info->tile_cells = 64u * ctx->C;
+    info->owned_cells = ctx->owned;
+    info->num_tiles = ctx->num_tiles;
+    uint64_t total = 0;
+    for (auto& r : ctx->runs) {
+        total += r.cells;
+    }
+    info->total_cells = total;
+    info->launches = ctx->last_launches;
+    info->state_bytes = ctx->state_bytes;
+    info->graph_launches = ctx->graph_launches;
+    return MBH_OK;
+}
+
+void mbh_destroy(mbh_ctx* ctx) { delete ctx; }
+
+const char* mbh_last_error(const mbh_ctx* ctx)
+{
+    if (ctx) {
+        return ctx->err.c_str();
+    }
+    return g_last_error.c_str();
+}
+
+} /* extern "C" */

@@ -1,0 +1,48 @@
+/*
+ * Kernel instantiations: noop (pure Maxwell) and regularized Cayley.
+ * FP64 storage implements MBH_FORM_FP64, FP32 storage MBH_FORM_MIXED.
+ */
+#include "mbh_tile.hpp"
+
+namespace mbh {
+
+#define MBH_ENTRY(ALGO_ID, LEV, FORM, C, A, ST)                               \
+    kernel_entry { ALGO_ID, LEV, FORM, C, &launch_tile<A, C, ST> }
+
+template __global__ void tile_kernel<algo_noop, 1, double>(KArgs);
+template __global__ void tile_kernel<algo_noop, 2, double>(KArgs);
+template __global__ void tile_kernel<algo_noop, 4, double>(KArgs);
+template __global__ void tile_kernel<algo_noop, 1, float>(KArgs);
+template __global__ void tile_kernel<algo_noop, 2, float>(KArgs);
+template __global__ void tile_kernel<algo_noop, 4, float>(KArgs);
+template __global__ void tile_kernel<algo_cayley<2>, 1, double>(KArgs);
+template __global__ void tile_kernel<algo_cayley<2>, 2, double>(KArgs);
+template __global__ void tile_kernel<algo_cayley<2>, 4, double>(KArgs);
+template __global__ void tile_kernel<algo_cayley<2>, 1, float>(KArgs);
+template __global__ void tile_kernel<algo_cayley<2>, 2, float>(KArgs);
+template __global__ void tile_kernel<algo_cayley<2>, 4, float>(KArgs);
+template __global__ void tile_kernel<algo_cayley<3>, 1, double>(KArgs);
+template __global__ void tile_kernel<algo_cayley<3>, 1, float>(KArgs);
+
+#if !defined(__HIP_DEVICE_COMPILE__)
+
+const kernel_entry k_table_basic[] = {
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_FP64, 1, algo_noop, double),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_FP64, 2, algo_noop, double),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_FP64, 4, algo_noop, double),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 1, algo_noop, float),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 2, algo_noop, float),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 4, algo_noop, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 1, algo_cayley<2>, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 2, algo_cayley<2>, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 4, algo_cayley<2>, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 1, algo_cayley<2>, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 2, algo_cayley<2>, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 4, algo_cayley<2>, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_FP64, 1, algo_cayley<3>, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_MIXED, 1, algo_cayley<3>, float),
+};
+const uint32_t k_table_basic_n = sizeof(k_table_basic) / sizeof(k_table_basic[0]);
+#endif
+
+} /* namespace mbh */

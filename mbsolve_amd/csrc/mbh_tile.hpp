@@ -1,0 +1,327 @@
+/*
+ * Temporally blocked FDTD + density tile kernel for gfx950.
+ *
+ * Geometry.  All runs of a context are laid out back to back in one
+ * "super-grid" of G cells: run r occupies cells [base_r, base_r + N_r)
+ * followed by one pseudo cell that holds the boundary value h[N_r] (always
+ * zero).  A cell flagged `hzero` has its H forced to zero after every H
+ * update (a run's first cell: h[0] = 0; the pseudo cell: h[N] = 0,
+ * solver_cpu_fdtd.tpp:273-275), which decouples neighbouring runs exactly.
+ *
+ * One wavefront (64 lanes) owns one tile.  Lane l holds the C consecutive
+ * cells g = g0 + l*C + c in registers, where g0 = t*owned - halo.  The
+ * wavefront advances `nsteps <= halo` time steps without leaving the
+ * registers; after s steps the outermost s cells at each tile edge are
+ * stale, so only the `owned = 64*C - 2*halo` central cells are written
+ * back.  Every cell's arithmetic is independent of the tiling, so results
+ * do not depend on C or the temporal block.
+ *
+ * Step order inside the block is the plain one of solver_cpu_fdtd::run
+ * (solver_cpu_fdtd.tpp:230-325): E update, sources, H update with
+ * boundary condition, density update, record copy.
+ */
+#ifndef MBH_TILE_HPP
+#define MBH_TILE_HPP
+
+#include "mbh_device.hpp"
+
+namespace mbh {
+
+constexpr int FDTD_STRIDE = 5; /* fac_a fac_b fac_c gamma dx_inv */
+
+struct dev_source {
+    uint64_t gx;         /* super-grid cell */
+    const double* data;  /* n_t values */
+    uint32_t type;
+    uint32_t pad;
+};
+
+struct dev_record {
+    uint64_t gbegin; /* super-grid cell of position_idx */
+    uint64_t cols;
+    uint64_t rows;
+    double ratio;
+    double* dst; /* scratch + offset_real */
+    uint32_t type;
+    uint32_t row_idx;
+    uint32_t col_idx;
+    uint32_t pad;
+};
+
+struct KArgs {
+    const void* e_in;
+    const void* h_in;
+    const void* p_in;
+    const void* d_in;
+    void* e_out;
+    void* h_out;
+    void* p_out;
+    void* d_out;
+    uint64_t G;       /* cells in the super-grid */
+    uint64_t dstride; /* elements between density components */
+    const int32_t* mat;   /* [G], -1: pseudo cell */
+    const uint8_t* hzero; /* [G] */
+    const double* fdtd;   /* [nmat][FDTD_STRIDE] */
+    const double* qm;     /* [nmat][qstride] */
+    uint64_t qstride;
+    const dev_source* src;     /* sorted by gx */
+    const uint32_t* tile_src;  /* [2*num_tiles]: [lo, hi) */
+    const dev_record* rec;
+    const uint32_t* tile_rec_ptr; /* [num_tiles + 1] */
+    const uint32_t* tile_rec_idx;
+    const uint64_t* n_base; /* device: first step = *n_base + n_off */
+    uint64_t n_off;
+    uint32_t nsteps;
+    uint32_t halo;
+    uint32_t owned;
+    uint32_t pad;
+};
+
+template<typename ST>
+__device__ __forceinline__ double ld(const void* p, uint64_t i)
+{
+    return (double)(static_cast<const ST*>(p)[i]);
+}
+template<typename ST>
+__device__ __forceinline__ void st(void* p, uint64_t i, double v)
+{
+    static_cast<ST*>(p)[i] = (ST)v;
+}
+
+/* Runs `fn(material)` for every lane whose `m >= 0`, one distinct material
+ * at a time; inside fn the material index is wave-uniform (SGPR), so
+ * constant reads through it become scalar loads. */
+template<typename F>
+__device__ __forceinline__ void for_each_material(int32_t m, F&& fn)
+{
+    bool todo = m >= 0;
+    for (;;) {
+        unsigned long long mask = __ballot(todo);
+        if (mask == 0ull) {
+            break;
+        }
+        int leader = __builtin_ctzll(mask);
+        int32_t mm = __builtin_amdgcn_readlane(m, leader);
+        if (todo && m == mm) {
+            fn(mm);
+            todo = false;
+        }
+    }
+}
+
+template<class A>
+__device__ __forceinline__ double record_value(const KArgs& k, uint32_t type,
+                                               uint32_t row, uint32_t col,
+                                               int32_t m, double e, double h,
+                                               double p,
+                                               const typename A::state& d)
+{
+    const double* q = k.qm + (uint64_t)(m < 0 ? 0 : m) * k.qstride;
+    switch (type) {
+    case MBH_REC_ELECTRIC:
+        return e;
+    case MBH_REC_POLAR_DT:
+        return p;
+    case MBH_REC_MAGNETIC:
+        return h;
+    case MBH_REC_INVERSION:
+        return (m < 0 || k.qstride == 0) ? 0.0 : A::inversion(q, d);
+    default: /* density: only populations are recorded */
+        return (m < 0 || k.qstride == 0) ? 0.0 : A::population(q, d, row);
+    }
+}
+
+template<class A, int C, typename ST>
+__global__ __launch_bounds__(64) void tile_kernel(KArgs k)
+{
+    const int lane = threadIdx.x;
+    const uint64_t t = blockIdx.x;
+    const int64_t own_lo = (int64_t)(t * k.owned);
+    const int64_t own_hi =
+        ((own_lo + (int64_t)k.owned) < (int64_t)k.G) ? own_lo + k.owned
+                                                      : (int64_t)k.G;
+    const int64_t gl = own_lo - (int64_t)k.halo + (int64_t)lane * C;
+    const uint64_t n0 = *k.n_base + k.n_off;
+
+    double e[C], h[C], p[C];
+    double fa[C], fb[C], fc[C], fg[C], fx[C];
+    typename A::state d[C];
+    int32_t m[C];
+    bool hz[C];
+
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const int64_t g = gl + c;
+        const bool in = (g >= 0) && (g < (int64_t)k.G);
+        m[c] = in ? k.mat[g] : -1;
+        hz[c] = in ? (k.hzero[g] != 0) : true;
+        e[c] = in ? ld<ST>(k.e_in, g) : 0.0;
+        h[c] = in ? ld<ST>(k.h_in, g) : 0.0;
+        p[c] = in ? ld<ST>(k.p_in, g) : 0.0;
+#pragma unroll
+        for (int j = 0; j < A::DC; j++) {
+            d[c].v[j] = in ? ld<ST>(k.d_in, (uint64_t)j * k.dstride + g) : 0.0;
+        }
+        if (m[c] >= 0) {
+            const double* f = k.fdtd + (uint64_t)m[c] * FDTD_STRIDE;
+            fa[c] = f[0];
+            fb[c] = f[1];
+            fc[c] = f[2];
+            fg[c] = f[3];
+            fx[c] = f[4];
+        } else {
+            fa[c] = fb[c] = fc[c] = fg[c] = fx[c] = 0.0;
+        }
+    }
+
+    const uint32_t s_lo = k.tile_src[2 * t];
+    const uint32_t s_hi = k.tile_src[2 * t + 1];
+    const uint32_t r_lo = k.tile_rec_ptr[t];
+    const uint32_t r_hi = k.tile_rec_ptr[t + 1];
+
+    for (uint32_t s = 0; s < k.nsteps; s++) {
+        const uint64_t n = n0 + s;
+
+        /* E update (solver_cpu_fdtd.tpp:238-243) */
+        const double h_right = __shfl_down(h[0], 1);
+        double en[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const double hn = (c + 1 < C) ? h[(c + 1) % C] : h_right;
+            en[c] = (m[c] >= 0)
+                ? fa[c] * e[c] + fb[c] * (-fg[c] * p[c] + (hn - h[c]) * fx[c])
+                : 0.0;
+        }
+
+        /* sources, in list order (solver_cpu_fdtd.tpp:247-255) */
+        for (uint32_t si = s_lo; si < s_hi; si++) {
+            const dev_source S = k.src[si];
+            const int64_t off = (int64_t)S.gx - gl;
+            const double v = S.data[n];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                if (off == c) {
+                    en[c] = (S.type == MBH_SRC_HARD) ? v : en[c] + v;
+                }
+            }
+        }
+
+        /* H update and boundary condition (solver_cpu_fdtd.tpp:263-275) */
+        const double e_left = __shfl_up(en[C - 1], 1);
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const double ep = (c > 0) ? en[(c + C - 1) % C] : e_left;
+            h[c] = hz[c] ? 0.0 : h[c] + fc[c] * (en[c] - ep);
+        }
+
+        /* density update (solver_cpu_fdtd.tpp:278-285) */
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            e[c] = en[c];
+            if constexpr (A::DC > 0) {
+                for_each_material(m[c], [&](int32_t mm) {
+                    A::update(k.qm + (uint64_t)mm * k.qstride, d[c], e[c],
+                              p[c]);
+                });
+            }
+        }
+
+        /* record copy (solver_cpu_fdtd.tpp:288-325) */
+        for (uint32_t ri = r_lo; ri < r_hi; ri++) {
+            const dev_record R = k.rec[k.tile_rec_idx[ri]];
+            bool now = (n == 0);
+            if (!now) {
+                now = floor(R.ratio * (double)(n - 1)) !=
+                    floor(R.ratio * (double)n);
+            }
+            if (!now) {
+                continue;
+            }
+            const uint64_t row = (uint64_t)floor(R.ratio * (double)n);
+            if (row >= R.rows) {
+                continue;
+            }
+            if (R.type == MBH_REC_DENSITY && R.row_idx != R.col_idx) {
+                continue; /* coherences are not recorded */
+            }
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                const int64_t g = gl + c;
+                if (g >= own_lo && g < own_hi && g >= (int64_t)R.gbegin &&
+                    g < (int64_t)(R.gbegin + R.cols)) {
+                    R.dst[row * R.cols + (uint64_t)(g - (int64_t)R.gbegin)] =
+                        record_value<A>(k, R.type, R.row_idx, R.col_idx, m[c],
+                                        e[c], h[c], p[c], d[c]);
+                }
+            }
+        }
+    }
+
+    /* write back the owned cells */
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const int64_t g = gl + c;
+        if (g >= own_lo && g < own_hi) {
+            st<ST>(k.e_out, g, e[c]);
+            st<ST>(k.h_out, g, h[c]);
+            st<ST>(k.p_out, g, p[c]);
+#pragma unroll
+            for (int j = 0; j < A::DC; j++) {
+                st<ST>(k.d_out, (uint64_t)j * k.dstride + g, d[c].v[j]);
+            }
+        }
+    }
+}
+
+/* ---- launcher registry ------------------------------------------------- */
+typedef hipError_t (*tile_launch_fn)(const KArgs& k, uint64_t num_tiles,
+                                     hipStream_t stream);
+
+struct kernel_entry {
+    uint32_t algo;
+    uint32_t levels;
+    uint32_t form; /* MBH_FORM_* the kernel implements */
+    uint32_t cells_per_lane;
+    tile_launch_fn fn;
+};
+
+template<class A, int C, typename ST>
+hipError_t launch_tile(const KArgs& k, uint64_t num_tiles, hipStream_t stream)
+{
+    hipLaunchKernelGGL((tile_kernel<A, C, ST>), dim3((unsigned)num_tiles),
+                       dim3(64), 0, stream, k);
+    return hipGetLastError();
+}
+
+/* z-form state conversions (per cell, uniform material) */
+struct ZArgs {
+    const double* dref; /* reference coherence vectors, component-major */
+    void* y;            /* z-form state (ST), component-major */
+    uint64_t G;
+    uint64_t dstride;
+    const int32_t* mat;
+    const double* zq;
+    uint64_t zstride;
+};
+typedef hipError_t (*zconv_fn)(const ZArgs& a, hipStream_t stream);
+
+struct zconv_entry {
+    uint32_t levels;
+    uint32_t fp32;
+    zconv_fn to_y;   /* dref -> y */
+    zconv_fn from_y; /* y -> dref (written into dref) */
+};
+
+/* tables defined by the instantiation units */
+extern const kernel_entry k_table_basic[];
+extern const uint32_t k_table_basic_n;
+extern const kernel_entry k_table_rodr[];
+extern const uint32_t k_table_rodr_n;
+extern const kernel_entry k_table_zform[];
+extern const uint32_t k_table_zform_n;
+extern const zconv_entry z_table[];
+extern const uint32_t z_table_n;
+
+} /* namespace mbh */
+
+#endif

@@ -1,0 +1,481 @@
+"""ORACLE set-up restatement -- test infrastructure only.
+
+Independent numpy/scipy restatement of everything the reference solver
+constructor computes before the time loop
+(solver-cpu/include/mbsolve/solver-cpu/solver_cpu_fdtd.tpp:36-203), written
+from the reference sources, not from the product code:
+
+* grid and time step            internal/common_fdtd.hpp:52-123
+* FDTD coefficients             internal/common_fdtd.hpp:158-190
+* Lindblad relaxation           qm_description.hpp:166-236, :375-404
+* coherence-vector matrices     internal/coherence_vector_representation.hpp
+* Cayley constants              internal/algo_lindblad_reg_cayley.hpp:123-203
+* Rodrigues constants           internal/algo_lindblad_cvr_rodr.hpp:101-229
+* initial density               algo_*::get_density, cvr :341-394
+* source table                  solver_cpu_fdtd.tpp:184-202, source.hpp
+* record copy list              internal/copy_list_entry.hpp:127-171
+
+Eigen is not available, so its routines are replaced by library routines with
+the same definition: scipy.linalg.expm (Higham 2005 scaling and squaring, the
+algorithm of Eigen's MatrixBase::exp), scipy.linalg.schur(output='real')
+(LAPACK real Schur, standing in for Eigen::RealSchur) and numpy.linalg.solve
+(for FullPivLU::solve on an invertible matrix).  Agreement with Eigen's
+rounding is therefore not pinned; what is pinned is checked in tests/.
+
+The input is a plain "spec" dict (see mbsolve_amd/setups.py for the format);
+the output has the layout of mbsolve_amd.lib.build_problem so both can be
+compared field by field and fed to the C oracle loop.
+"""
+import math
+
+import numpy as np
+import scipy.linalg
+
+# physical constants (mbsolve-lib/include/mbsolve/lib/types.hpp:36-73)
+HBAR = 1.05457266e-34
+PI = math.atan(1.0) * 4.0
+MU0 = PI * 4e-7
+EPS0 = 8.854187817e-12
+E0 = 1.60217733e-19
+KB = 1.38064852e-23
+
+ALGO_IDS = {"noop": 0, "reg-cayley": 1, "cvr-rodr": 2}
+
+
+# ---------------------------------------------------------------- grid -----
+def init_fdtd_simulation(length, eps_min, num_gridpoints, endtime,
+                         num_timesteps=2, courant=1.0):
+    """common_fdtd.hpp:52-123 -> (dx, dt, n_t)."""
+    if num_gridpoints <= 0:
+        raise ValueError("Invalid scenario.")
+    velocity = 1.0 / math.sqrt(MU0 * EPS0 * eps_min)
+    if num_gridpoints == 1:
+        dx = 1e13
+        n_t = num_timesteps
+        if n_t < 2:
+            raise ValueError("Invalid scenario.")
+    else:
+        dx = length / (num_gridpoints - 1)
+        d_t = courant * dx / velocity
+        n_t = int(math.ceil(endtime / d_t) + 1)
+    if endtime > 2.2250738585072014e-308:
+        dt = endtime / (n_t - 1)
+    else:
+        n_t = 1
+        dt = 3.154e10
+    return dx, dt, n_t
+
+
+def fdtd_constants(mat, dt, dx):
+    """common_fdtd.hpp:158-190 -> (fac_a, fac_b, fac_c, gamma)."""
+    eps_r = mat.get("rel_permittivity", 1.0)
+    mu_r = mat.get("rel_permeability", 1.0)
+    losses = mat.get("losses", 0.0)
+    sigma = math.sqrt(EPS0 * eps_r / (MU0 * mu_r)) * losses * 2.0
+    epsilon = EPS0 * eps_r
+    x = (sigma * dt) / (2.0 * epsilon)
+    return ((1.0 - x) / (1.0 + x), dt / epsilon / (1.0 + x),
+            dt / (MU0 * mu_r * dx), mat.get("overlap_factor", 1.0))
+
+
+# ------------------------------------------------------ qm description -----
+def lindblad_relaxation(scattering, pure_dephasing):
+    """qm_lindblad_relaxation constructor (qm_description.hpp:166-204):
+    returns (s with s[m][m] = -sum_{j!=m} s[j][m], total dephasing rates)."""
+    s = np.array(scattering, dtype=float)
+    n = s.shape[0]
+    for m in range(n):
+        s[m, m] = -sum(s[j, m] for j in range(n) if j != m)
+    deph = list(pure_dephasing)
+    idx = 0
+    for col in range(n):
+        for row in range(col):
+            deph[idx] = -abs(deph[idx]) - 0.5 * (abs(s[row, row]) +
+                                                 abs(s[col, col]))
+            idx += 1
+    return s, np.array(deph[:n * (n - 1) // 2] if n > 1 else [], float)
+
+
+def qm_from_spec(q):
+    """Normalises a qm spec: either {'type': '2lvl', ...} (qm_desc_2lvl,
+    qm_description.hpp:375-404) or a general description."""
+    if q.get("type") == "2lvl":
+        w0, d = q["transition_freq"], q["dipole_moment"]
+        g, G = q["scattering_rate"], q["dephasing_rate"]
+        winv = q.get("equilibrium_inversion", -1.0)
+        return {
+            "num_levels": 2,
+            "carrier_density": q["carrier_density"],
+            "h_diag": [-HBAR * w0 / 2, +HBAR * w0 / 2],
+            "h_off": [],
+            "mu_diag": [0.0, 0.0],
+            "mu_off": [-E0 * d],
+            "scattering": [[0.0, g * 0.5 * (1 - winv)],
+                           [g * 0.5 * (1 + winv), 0.0]],
+            "pure_dephasing": [G - 0.5 * g],
+        }
+    return q
+
+
+def to_matrix(diag, off, n):
+    """Hermitian matrix of a qm_operator: off-diagonal entries in
+    column-major upper-triangle order (algo_lindblad_reg_cayley.hpp:91-121)."""
+    m = np.zeros((n, n), dtype=complex)
+    for i in range(n):
+        if i < len(diag):
+            m[i, i] = diag[i]
+    row, col = 0, 1
+    for v in off:
+        if col >= n:
+            break
+        m[row, col] = v
+        m[col, row] = np.conj(v)
+        if row == col - 1:
+            row, col = 0, col + 1
+        else:
+            row += 1
+    return m
+
+
+def relaxation_action(s, deph, op):
+    """qm_lindblad_relaxation::action (qm_description.hpp:211-236) on a
+    Hermitian matrix."""
+    n = op.shape[0]
+    out = np.zeros_like(op)
+    pops = s @ np.real(np.diag(op))
+    for m in range(n):
+        out[m, m] = pops[m]
+    idx = 0
+    for col in range(n):
+        for row in range(col):
+            out[row, col] = op[row, col] * deph[idx]
+            out[col, row] = np.conj(out[row, col])
+            idx += 1
+    return out
+
+
+def generators(n):
+    """su(N) generators in the reference order
+    (coherence_vector_representation.hpp:52-93)."""
+    gens = []
+    for k in range(n):
+        for j in range(k):
+            g = np.zeros((n, n), complex)
+            g[j, k] = g[k, j] = 1.0
+            gens.append(g)
+    for k in range(n):
+        for j in range(k):
+            g = np.zeros((n, n), complex)
+            g[j, k] = -1j
+            g[k, j] = +1j
+            gens.append(g)
+    for l in range(n - 1):
+        g = np.zeros((n, n), complex)
+        for j in range(l + 1):
+            g[j, j] = -1.0
+        g[l + 1, l + 1] = l + 1
+        g /= math.sqrt(((l + 1) * (l + 2)) // 2)
+        gens.append(g)
+    return gens
+
+
+def cv_matrices(q):
+    """cv_representation (coherence_vector_representation.hpp:149-241):
+    Liouvillians of H and mu, mu vector, relaxation superoperator,
+    equilibrium vector."""
+    n = q["num_levels"]
+    gens = generators(n)
+    K = n * n - 1
+    H = to_matrix(q["h_diag"], q["h_off"], n)
+    mu = to_matrix(q["mu_diag"], q["mu_off"], n)
+    s, deph = lindblad_relaxation(q["scattering"], q["pure_dephasing"])
+
+    def liouvillian(op):
+        L = np.zeros((K, K))
+        for i in range(K):
+            for j in range(K):
+                c = 1j * 0.5 * np.trace(op @ (gens[i] @ gens[j] -
+                                              gens[j] @ gens[i]))
+                L[i, j] = c.real / HBAR
+        return L
+
+    G = np.zeros((K, K))
+    for j in range(K):
+        rg = relaxation_action(s, deph, gens[j])
+        for i in range(K):
+            G[i, j] = (0.5 * np.trace(rg @ gens[i])).real
+    rI = relaxation_action(s, deph, np.eye(n, dtype=complex))
+    d_eq = np.array([np.trace(rI @ g).real / n for g in gens])
+    v = np.array([np.trace(mu @ g).real for g in gens])
+    return {"hamiltonian": liouvillian(H), "dipole_operator": liouvillian(mu),
+            "dipole_operator_vec": v, "relaxation_superop": G,
+            "equilibrium_vec": d_eq, "s": s, "deph": deph}
+
+
+# ------------------------------------------------------------- cayley -----
+def cayley_constants(q, dt, n):
+    """lindblad_reg_cayley<N>::get_qm_constants
+    (algo_lindblad_reg_cayley.hpp:123-203) in the MBH_CAYLEY_STRIDE layout."""
+    blob = np.zeros(5 + 8 * n * n)
+    blob[2] = dt
+    if q is None:
+        return blob
+    q = qm_from_spec(q)
+    if q["num_levels"] != n:
+        raise ValueError("Number of energy levels does not match selected "
+                         "solver!")
+    s, deph = lindblad_relaxation(q["scattering"], q["pure_dephasing"])
+    H0 = to_matrix(q["h_diag"], q["h_off"], n)
+    mu = to_matrix(q["mu_diag"], q["mu_off"], n)
+    d1 = s.copy()
+    D1 = scipy.linalg.expm(d1 * dt / 2)
+    d2 = np.zeros((n, n))
+    row, col = 0, 1
+    for r in deph:
+        d2[row, col] = d2[col, row] = r
+        if row == col - 1:
+            row, col = 0, col + 1
+        else:
+            row += 1
+    D2 = np.exp(d2 * dt / 2)
+    blob[0] = 1.0
+    blob[1] = q["carrier_density"]
+    blob[3] = (-1.0 / HBAR) * dt
+    blob[4] = -1.0 / HBAR
+    o = 5
+    blob[o:o + 2 * n * n] = np.column_stack([H0.real.ravel(),
+                                             H0.imag.ravel()]).ravel()
+    o += 2 * n * n
+    blob[o:o + 2 * n * n] = np.column_stack([mu.real.ravel(),
+                                             mu.imag.ravel()]).ravel()
+    o += 2 * n * n
+    for m in (d1, d2, D1, D2):
+        blob[o:o + n * n] = m.ravel()
+        o += n * n
+    return blob
+
+
+# ---------------------------------------------------------- rodrigues -----
+def rodr_constants(q, dt, n):
+    """lindblad_cvr_rodr<N>::get_qm_constants and fill_coefficients
+    (algo_lindblad_cvr_rodr.hpp:101-229) in the MBH_RODR_STRIDE layout.
+
+    The Rodrigues planes come from the real Schur form of U, sorted by
+    |T(i, i-1)| in descending order, the first floor(K/2) kept, each with
+    theta = T(i, i-1), C1 = Z B Z^T, C2 = Z B B Z^T (B(i,i-1) = 1,
+    B(i-1,i) = -1).  Planes with theta == 0 add exact zeros in the reference
+    and are left out of the blob."""
+    K = n * n - 1
+    B = K // 2
+    KK = K * K
+    blob = np.zeros(5 + 4 * KK + 3 * K + B + 2 * B * KK)
+    if q is None:
+        return blob, None
+    q = qm_from_spec(q)
+    if q["num_levels"] != n:
+        raise ValueError("Number of energy levels does not match selected "
+                         "solver!")
+    cv = cv_matrices(q)
+    M = cv["hamiltonian"] + cv["relaxation_superop"]
+    U = -cv["dipole_operator"]
+    U2 = U @ U
+    theta_1 = math.sqrt(U[0, 1] ** 2 + U[0, 2] ** 2 + U[1, 2] ** 2)
+    A1 = scipy.linalg.expm(M * dt / 2)
+    T, Z = scipy.linalg.schur(U, output="real")
+    idx = sorted(range(1, K), key=lambda i: -abs(T[i, i - 1]))[:B]
+    thetas, C1s, C2s = [], [], []
+    for i1 in idx:
+        th = T[i1, i1 - 1]
+        if th == 0.0:
+            continue
+        b = np.zeros((K, K))
+        b[i1, i1 - 1] = 1.0
+        b[i1 - 1, i1] = -1.0
+        thetas.append(th)
+        C1s.append(Z @ b @ Z.T)
+        C2s.append(Z @ b @ b @ Z.T)
+    d_eq = cv["equilibrium_vec"]
+    if np.all(np.abs(d_eq) <= np.finfo(float).eps):
+        d_in = np.zeros(K)
+    else:
+        d_in = np.linalg.solve(M, d_eq)
+        err = np.linalg.norm(M @ d_in - d_eq) / np.linalg.norm(d_eq)
+        if err > 1e-3:
+            raise ValueError("Time-indepent matrix not invertible!")
+    blob[0] = 1.0
+    blob[1] = 0.5 * q["carrier_density"]
+    blob[2] = dt
+    blob[3] = theta_1
+    blob[4] = len(thetas)
+    o = 5
+    for m in (M, U, U2, A1):
+        blob[o:o + KK] = m.ravel()
+        o += KK
+    for vec in (cv["dipole_operator_vec"], d_eq, d_in):
+        blob[o:o + K] = vec
+        o += K
+    blob[o:o + len(thetas)] = thetas
+    o += B
+    for j, c in enumerate(C1s):
+        blob[o + j * KK:o + (j + 1) * KK] = c.ravel()
+    o += B * KK
+    for j, c in enumerate(C2s):
+        blob[o + j * KK:o + (j + 1) * KK] = c.ravel()
+    extra = {"M": M, "U": U, "A1": A1, "d_in": d_in, "d_eq": d_eq,
+             "v": cv["dipole_operator_vec"], "M_CP": 0.5 *
+             q["carrier_density"], "thetas": np.array(thetas)}
+    return blob, extra
+
+
+def coherence_vector(diag, off, n):
+    """cv_representation::coherence_vector
+    (coherence_vector_representation.hpp:341-394), including its behaviour
+    of never advancing the output index in the off-diagonal loops."""
+    K = n * n - 1
+    d = np.zeros(K)
+    for v in off:
+        d[0] = 2 * np.real(v)
+    for v in off:
+        d[(n * (n - 1)) // 2] = -2 * np.imag(v)
+    for l in range(n - 1):
+        entry = 0.0
+        for j in range(l + 1):
+            entry += -1.0 * diag[j]
+        entry += (l + 1) * diag[l + 1]
+        d[n * (n - 1) + l] = entry / math.sqrt(((l + 1) * (l + 2)) // 2)
+    return d
+
+
+def cv_population(d, n, m):
+    """cv_representation::calc_population (:289-336)."""
+    if n == 2:
+        return 0.5 - 0.5 * d[2] if m == 0 else 0.5 + 0.5 * d[2]
+    if n == 3:
+        s3 = math.sqrt(3.0)
+        return [1.0 / 3 + 0.5 * (-1.0 * d[6] - 1.0 / s3 * d[7]),
+                1.0 / 3 + 0.5 * (+1.0 * d[6] - 1.0 / s3 * d[7]),
+                1.0 / 3 + 0.5 * (+2.0 / s3 * d[7])][m] if m < 3 else 0.0
+    total = 0.0
+    for i in range(n - 1):
+        factor = -1.0 if m < i + 1 else (float(m) if m == i + 1 else 0.0)
+        total += factor / math.sqrt(((i + 1) * (i + 2)) // 2) * \
+            d[n * (n - 1) + i]
+    return 1.0 / n + 0.5 * total
+
+
+# ------------------------------------------------------------ sources -----
+def source_value(src, t):
+    """source::get_value with sech_pulse / gaussian_pulse / cw_source
+    (source.hpp:94-106, :190-194, :256-260)."""
+    kind = src["kind"]
+    if kind == "sech":
+        v = 1 / math.cosh(src["beta"] * t - src["phase"]) * \
+            math.sin(2 * PI * src["freq"] * t - src.get("phase_sin", 0.0))
+    elif kind == "gauss":
+        v = math.exp(-(t - src["phase"]) * (t - src["phase"]) /
+                     (src["tau"] * src["tau"])) * \
+            math.sin(2 * PI * src["freq"] * t)
+    elif kind == "cw":
+        v = math.sin(2 * PI * src["freq"] * t + src.get("phase", 0.0))
+    else:
+        raise ValueError("unknown source kind " + kind)
+    return src["ampl"] * v
+
+
+# ------------------------------------------------------------ problem -----
+def build_problem(spec):
+    """Full host set-up of one run from a spec dict; the output matches the
+    dict of mbsolve_amd.lib.build_problem."""
+    algo_name = spec["algo"]
+    algo = ALGO_IDS[algo_name]
+    n = spec.get("num_levels", 0) if algo else 0
+    regions = spec["regions"]
+    mats = spec["materials"]
+    if not regions:
+        raise ValueError("No regions in device!")
+    length = sum(r[3] - r[2] for r in regions)
+    eps_min = min(mats[r[1]].get("rel_permittivity", 1.0) for r in regions)
+    dx, dt, n_t = init_fdtd_simulation(
+        length, eps_min, spec["num_gridpoints"], spec["endtime"],
+        spec.get("num_timesteps", 2), courant=0.5)
+    nx = spec["num_gridpoints"]
+    used = sorted({r[1] for r in regions})
+    id_to_idx = {m: i for i, m in enumerate(used)}
+    fdtd, qm = [], []
+    for mid in used:
+        mat = mats[mid]
+        fdtd.extend(fdtd_constants(mat, dt, dx))
+        q = mat.get("qm")
+        if algo == 1:
+            qm.append(cayley_constants(q, dt, n))
+        elif algo == 2:
+            qm.append(rodr_constants(q, dt, n)[0])
+    ds = 2 * n * n if algo == 1 else (n * n - 1 if algo == 2 else 0)
+    mat_idx = np.zeros(nx, np.uint32)
+    d_init = np.zeros(nx * ds)
+    rho_diag, rho_off = spec["rho_init"]
+    if algo == 1:
+        m = to_matrix(rho_diag, rho_off, n)
+        d_qm = np.column_stack([m.real.ravel(), m.imag.ravel()]).ravel()
+    elif algo == 2:
+        d_qm = coherence_vector(rho_diag, rho_off, n)
+    for i in range(nx):
+        x = i * dx
+        idx, has_qm = -1, False
+        for r in regions:
+            if r[2] <= x <= r[3]:
+                idx = id_to_idx[r[1]]
+                has_qm = mats[r[1]].get("qm") is not None
+                break
+        if idx < 0:
+            raise ValueError("region not found")
+        mat_idx[i] = idx
+        if ds and has_qm:
+            d_init[i * ds:(i + 1) * ds] = d_qm
+    e_init = np.full(nx, float(spec.get("e_init", 0.0)))
+    h_init = np.full(nx + 1, float(spec.get("h_init", 0.0)))
+    h_init[nx] = 0.0
+    records, scratch = [], 0
+    for rec in spec.get("records", []):
+        name, interval, position = rec["name"], rec.get("interval", 0.0), \
+            rec.get("position", -1.0)
+        rtype = rec.get("type", {"e": 0, "p": 1, "h": 2, "d": 3,
+                                 "i": 4}[name[0]])
+        row = rec.get("row", 1) - 1 if "row" in rec else 0
+        col = rec.get("col", 1) - 1 if "col" in rec else 0
+        if interval <= dt:
+            rows, ratio = n_t, 1.0
+        else:
+            rows = int(math.floor(spec["endtime"] / interval) + 1)
+            ratio = dt / interval
+        if position < 0.0:
+            pos, cols = 0, nx
+        else:
+            pos, cols = int(round(position / dx)), 1
+        is_complex = int(rtype == 3 and row != col)
+        records.append({"type": rtype, "row_idx": row, "col_idx": col,
+                        "is_complex": is_complex, "rows": rows, "cols": cols,
+                        "position_idx": pos, "interval_ratio": ratio,
+                        "offset_real": scratch,
+                        "offset_imag": scratch + rows * cols})
+        scratch += rows * cols * (2 if is_complex else 1)
+    stype, sx, sdata = [], [], []
+    for src in spec.get("sources", []):
+        stype.append(0 if src.get("hard", True) else 1)
+        sx.append(int(src["position"] / dx))
+        sdata.extend(source_value(src, j * dt) for j in range(n_t))
+    return {
+        "algo": algo, "num_levels": n, "num_materials": len(used),
+        "num_gridpoints": nx, "num_timesteps": n_t, "dx_inv": 1.0 / dx,
+        "fdtd_consts": np.array(fdtd, float),
+        "qm_consts": np.concatenate(qm) if qm else np.zeros(0),
+        "qm_stride": len(qm[0]) if qm else 0,
+        "zform_consts": np.zeros(0), "zform_stride": 0,
+        "mat_idx": mat_idx, "e_init": e_init, "h_init": h_init,
+        "d_init": d_init,
+        "source_type": np.array(stype, np.uint32),
+        "source_x": np.array(sx, np.uint32),
+        "source_data": np.array(sdata, float),
+        "records": records, "scratch_size": scratch,
+        "material_ids": used, "dx": dx, "dt": dt,
+    }

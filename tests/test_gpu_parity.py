@@ -1,0 +1,99 @@
+"""GPU parity: the HIP solvers against the CPU FP64 oracle on identical
+host-computed constants (same mbh_problem), through the solver factory and
+directly through the C ABI."""
+import numpy as np
+import pytest
+
+import mbsolve_amd.lib as mb
+import mbsolve_amd.solverhip  # noqa: F401  (registers hip-fdtd-*)
+from mbsolve_amd import capi, setups
+import oracle
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12   # north-star bound for the FP64 form
+MIXED_TOL = 1e-5   # bound for the FP32-storage form
+
+
+def _run_hip(spec, form="", **opts):
+    dev, scen = setups.build(spec)
+    name = setups.solver_name(spec, form)
+    print("case", spec["name"], name, opts, flush=True)
+    sol = mb.create_hip_solver(name, dev, scen, **opts)
+    sol.run()
+    return sol, [r.get_data_real() for r in sol.get_results()]
+
+
+def _oracle_records(sol):
+    return oracle.OracleRun(sol.problem()).run().records()
+
+
+SMALL = [
+    ("sit2", lambda: setups.sit_2lvl(num_gridpoints=2048, endtime=0.9e-12,
+                                     num_records=32)),
+    ("v3", lambda: setups.v3_rodr(num_gridpoints=2048, endtime=0.9e-12,
+                                  num_records=32)),
+    ("qcl6", lambda: setups.qcl6(num_gridpoints=512, num_timesteps=400,
+                                 num_records=8)),
+    ("zio", lambda: setups.ziolkowski1995(num_gridpoints=2048,
+                                          endtime=60e-15)),
+]
+
+
+@pytest.mark.parametrize("case", [c[0] for c in SMALL])
+def test_fp64_matches_oracle(gpu, case):
+    spec = dict(SMALL)[case]()
+    sol, recs = _run_hip(spec)
+    ref = _oracle_records(sol)
+    for a, b in zip(recs, ref):
+        assert a.shape == b.shape
+        assert rel_err(a, b) <= FP64_TOL, (case, rel_err(a, b))
+
+
+@pytest.mark.parametrize("tb,tile", [(1, 64), (5, 128), (16, 256), (32, 256)])
+def test_fp64_independent_of_tiling(gpu, tb, tile):
+    spec = setups.sit_2lvl(num_gridpoints=1500, endtime=0.5e-12,
+                           num_records=16)
+    _, base = _run_hip(spec, temporal_block=1, tile_cells=64)
+    _, recs = _run_hip(spec, temporal_block=tb, tile_cells=tile)
+    for a, b in zip(recs, base):
+        assert np.array_equal(a, b)
+
+
+def test_mixed_rodr_within_tolerance(gpu):
+    spec = setups.v3_rodr(num_gridpoints=2048, endtime=0.9e-12,
+                          num_records=32)
+    sol, recs = _run_hip(spec, "mixed")
+    ref = _oracle_records(sol)
+    for a, b in zip(recs, ref):
+        assert rel_err(a, b) <= MIXED_TOL, rel_err(a, b)
+
+
+def test_zform64_close_to_reference_form(gpu):
+    spec = setups.v3_rodr(num_gridpoints=2048, endtime=0.9e-12,
+                          num_records=32)
+    sol, recs = _run_hip(spec, "zform")
+    ref = _oracle_records(sol)
+    for a, b in zip(recs, ref):
+        assert rel_err(a, b) <= 1e-10, rel_err(a, b)
+
+
+def test_capi_direct(gpu):
+    spec = setups.sit_2lvl(num_gridpoints=1000, endtime=0.3e-12,
+                           num_records=8)
+    dev, scen = setups.build(spec)
+    prob = mb.build_problem("hip-fdtd-2lvl-reg-cayley", dev, scen)
+    ctx = capi.Context([prob], capi.options("fp64", temporal_block=8))
+    nt = int(prob["num_timesteps"])
+    ctx.run(0, nt // 2)
+    ctx.run(nt // 2, nt)
+    scratch = ctx.records(0)
+    ref = oracle.OracleRun(prob).run()
+    assert np.array_equal(scratch, ref.scratch[:len(scratch)])
+    st = ctx.state(0)
+    rs = ref.state()
+    for k in ("e", "h", "p", "d"):
+        assert np.array_equal(st[k], rs[k]), k
+    ctx.close()
