@@ -83,6 +83,11 @@ struct mbh_ctx {
     double* scratch = nullptr;
     uint64_t* n_base = nullptr;
 
+    /* host copies for rebuilding per-geometry tile tables */
+    std::vector<dev_source> srcs;
+    std::vector<dev_record> recs;
+    uint64_t tuned_candidates = 0;
+
     /* graphs */
     uint32_t graph_launches = 1;
     hipGraphExec_t graph[2] = { nullptr, nullptr }; /* per start buffer */
@@ -328,63 +333,186 @@ uint32_t kernel_form(uint32_t algo, uint32_t form)
     return form;
 }
 
-/* Tile geometry: C cells per lane, halo = temporal block.  Prefers the
- * largest temporal block whose halo overhead stays below ~25% while
- * keeping at least ~4 wavefronts per CU busy. */
-void choose_geometry(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
+/* Cells owned by tiles: the trailing pseudo cell is never written. */
+uint64_t owned_cells(const mbh_ctx* ctx) { return ctx->G - 1; }
+
+void set_geometry(mbh_ctx* ctx, uint32_t C, uint32_t tb)
 {
-    const uint32_t algo = ctx->algo, levels = ctx->levels;
-    const uint32_t kform = kernel_form(algo, ctx->form);
+    ctx->C = C;
+    ctx->tb = tb;
+    ctx->halo = tb;
+    ctx->owned = 64u * C - 2u * tb;
+    ctx->num_tiles = (owned_cells(ctx) + ctx->owned - 1) / ctx->owned;
+}
+
+/* Per-tile source ranges (sources inside the loaded span, halo included)
+ * and record lists (records overlapping the owned span). */
+int build_tile_tables(mbh_ctx* ctx, bool with_records)
+{
+    const uint64_t nt = ctx->num_tiles;
+    std::vector<uint32_t> ts(2 * nt);
+    for (uint64_t t = 0; t < nt; t++) {
+        const int64_t lo = (int64_t)(t * ctx->owned) - ctx->halo;
+        const int64_t hi = lo + 64 * (int64_t)ctx->C;
+        auto a = std::lower_bound(ctx->srcs.begin(), ctx->srcs.end(), lo,
+                                  [](const dev_source& s, int64_t v) {
+                                      return (int64_t)s.gx < v;
+                                  });
+        auto b = std::lower_bound(ctx->srcs.begin(), ctx->srcs.end(), hi,
+                                  [](const dev_source& s, int64_t v) {
+                                      return (int64_t)s.gx < v;
+                                  });
+        ts[2 * t] = (uint32_t)(a - ctx->srcs.begin());
+        ts[2 * t + 1] = (uint32_t)(b - ctx->srcs.begin());
+    }
+    std::vector<uint32_t> ptr(nt + 1, 0), idx;
+    if (with_records) {
+        std::vector<std::pair<uint64_t, uint32_t>> tile_rec;
+        for (uint32_t id = 0; id < ctx->recs.size(); id++) {
+            const dev_record& d = ctx->recs[id];
+            const uint64_t t0 = d.gbegin / ctx->owned;
+            const uint64_t t1 = (d.gbegin + d.cols - 1) / ctx->owned;
+            for (uint64_t t = t0; t <= t1 && t < nt; t++) {
+                tile_rec.emplace_back(t, id);
+            }
+        }
+        std::stable_sort(tile_rec.begin(), tile_rec.end(),
+                         [](const std::pair<uint64_t, uint32_t>& x,
+                            const std::pair<uint64_t, uint32_t>& y) {
+                             return x.first < y.first;
+                         });
+        idx.reserve(tile_rec.size());
+        for (auto& tr : tile_rec) {
+            ptr[tr.first + 1]++;
+            idx.push_back(tr.second);
+        }
+        for (uint64_t t = 0; t < nt; t++) {
+            ptr[t + 1] += ptr[t];
+        }
+    }
+    /* replace the previous tables */
+    auto release = [ctx](void* p) {
+        if (!p) {
+            return;
+        }
+        auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), p);
+        if (it != ctx->allocs.end()) {
+            ctx->allocs.erase(it);
+        }
+        (void)hipFree(p);
+    };
+    release(ctx->tile_src);
+    release(ctx->tile_rec_ptr);
+    release(ctx->tile_rec_idx);
+    ctx->tile_src = ctx->tile_rec_ptr = ctx->tile_rec_idx = nullptr;
+    int rc;
+    if ((rc = upload(ctx, &ctx->tile_src, ts.data(), ts.size()))) {
+        return rc;
+    }
+    if ((rc = upload(ctx, &ctx->tile_rec_ptr, ptr.data(), ptr.size()))) {
+        return rc;
+    }
+    return upload(ctx, &ctx->tile_rec_idx, idx.data(), idx.size());
+}
+
+KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
+                uint32_t nsteps);
+
+/*
+ * Autotuner: times every admissible (cells per lane, temporal block) pair
+ * on the context's own state and picks the fastest per cell update.  The
+ * trial launches read the current buffer, write the other one (discarded)
+ * and record nothing, so the simulation state is untouched.
+ */
+int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
+{
+    const uint32_t kform = kernel_form(ctx->algo, ctx->form);
     std::vector<uint32_t> cs;
-    for (uint32_t C : { 4u, 2u, 1u }) {
-        if (find_kernel(algo, levels, kform, C)) {
+    for (uint32_t C : { 1u, 2u, 4u }) {
+        if (find_kernel(ctx->algo, ctx->levels, kform, C) &&
+            (!req_tile || req_tile == 64u * C)) {
             cs.push_back(C);
         }
     }
-    if (req_tile) {
-        uint32_t C = std::max(1u, req_tile / 64u);
-        bool ok = std::find(cs.begin(), cs.end(), C) != cs.end();
-        cs.assign(1, ok ? C : cs.back());
+    if (cs.empty()) {
+        return fail(ctx, MBH_ERR_UNSUPPORTED,
+                    "no kernel with %u cells per wavefront", req_tile);
     }
-    const double target_tiles = 1024.0;
-    double best = -1.0;
-    uint32_t bc = cs.back(), btb = 1;
+    std::vector<std::pair<uint32_t, uint32_t>> cand;
     for (uint32_t C : cs) {
-        const uint32_t span = 64u * C;
-        for (uint32_t tb : { 64u, 48u, 32u, 24u, 16u, 12u, 8u, 6u, 4u, 2u,
-                             1u }) {
+        for (uint32_t tb : { 1u, 2u, 4u, 6u, 8u, 12u, 16u, 20u, 24u, 32u,
+                             48u }) {
             if (req_tb && tb != req_tb) {
                 continue;
             }
-            if (2u * tb >= span) {
+            if (tb > MAX_BLOCK || 64u * C < 2u * tb + 16u) {
                 continue;
             }
-            const uint32_t own = span - 2u * tb;
-            const double tiles = std::ceil((double)ctx->G / own);
-            const double eff = (double)own / span;
-            /* fraction of the machine kept busy */
-            const double fill = std::min(1.0, tiles / target_tiles);
-            /* launch amortisation: a launch costs roughly as much as a few
-             * microseconds of work */
-            const double amort = tb / (tb + 2.0);
-            const double score = eff * fill * amort;
-            if (score > best) {
-                best = score;
-                bc = C;
-                btb = tb;
+            if (!req_tb && tb > std::max<uint64_t>(1, ctx->n_t)) {
+                continue;
             }
+            cand.emplace_back(C, tb);
         }
     }
-    if (req_tb && best < 0.0) {
-        /* requested block too large for every tile: use the widest tile */
-        bc = cs.front();
-        btb = std::min(req_tb, 32u * bc - 1u);
+    if (cand.empty()) {
+        /* requested block too deep for any tile: widest tile, clamp */
+        const uint32_t C = cs.back();
+        cand.emplace_back(C, std::min<uint32_t>(req_tb, 32u * C - 8u));
     }
-    ctx->C = bc;
-    ctx->tb = btb;
-    ctx->halo = btb;
-    ctx->owned = 64u * bc - 2u * btb;
-    ctx->num_tiles = (ctx->G + ctx->owned - 1) / ctx->owned;
+    if (cand.size() == 1) {
+        set_geometry(ctx, cand[0].first, cand[0].second);
+        ctx->launch =
+            find_kernel(ctx->algo, ctx->levels, kform, ctx->C)->fn;
+        return MBH_OK;
+    }
+    /* cheapest candidates (shallow blocks) first would bias the budget;
+     * the list is tried in order until ~0.5 s of trial launches are spent */
+    double best = 1e300, spent_ms = 0.0;
+    std::pair<uint32_t, uint32_t> pick = cand[0];
+    size_t tried = 0;
+    hipLaunchKernelGGL(set_u64, dim3(1), dim3(1), 0, ctx->stream,
+                       ctx->n_base, (uint64_t)0);
+    for (auto& cd : cand) {
+        if (spent_ms > 500.0) {
+            break;
+        }
+        set_geometry(ctx, cd.first, cd.second);
+        ctx->launch = find_kernel(ctx->algo, ctx->levels, kform, cd.first)->fn;
+        int rc = build_tile_tables(ctx, false);
+        if (rc) {
+            return rc;
+        }
+        const uint32_t steps = (uint32_t)std::min<uint64_t>(cd.second,
+                                                            ctx->n_t);
+        KArgs k = make_args(ctx, ctx->cur, 0, steps);
+        /* one warm-up launch, timed */
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        HIP_TRY(ctx, ctx->launch(k, ctx->num_tiles, ctx->stream));
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        float ms0 = 0.0f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms0, ctx->ev0, ctx->ev1));
+        const int reps = (ms0 > 20.0f) ? 1 : 4;
+        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+        for (int i = 0; i < reps; i++) {
+            HIP_TRY(ctx, ctx->launch(k, ctx->num_tiles, ctx->stream));
+        }
+        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+        float ms = 0.0f;
+        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        spent_ms += ms0 + ms;
+        tried++;
+        const double per_step = (double)ms / (reps * (double)steps);
+        if (per_step < best) {
+            best = per_step;
+            pick = cd;
+        }
+    }
+    ctx->tuned_candidates = tried;
+    set_geometry(ctx, pick.first, pick.second);
+    ctx->launch = find_kernel(ctx->algo, ctx->levels, kform, ctx->C)->fn;
+    return MBH_OK;
 }
 
 KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
@@ -401,6 +529,7 @@ KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
     k.p_out = ctx->state[ob][2];
     k.d_out = ctx->state[ob][3];
     k.G = ctx->G;
+    k.Gown = owned_cells(ctx);
     k.dstride = ctx->dstride;
     k.mat = ctx->mat;
     k.hzero = ctx->hzero;
@@ -595,15 +724,11 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
      * parity is the same before and after a replay */
     c->graph_launches =
         (opt.graph_launches > 1) ? ((opt.graph_launches + 1u) & ~1u) : 1u;
-    choose_geometry(c, opt.temporal_block, opt.tile_cells);
-    const kernel_entry* ke =
-        find_kernel(c->algo, c->levels, kernel_form(c->algo, c->form), c->C);
-    if (!ke) {
+    if (!find_kernel(c->algo, c->levels, kernel_form(c->algo, c->form), 1)) {
         return fail(nullptr, MBH_ERR_UNSUPPORTED,
                     "no kernel for algo %u, %u levels, form %u", c->algo,
                     c->levels, c->form);
     }
-    c->launch = ke->fn;
     if (c->zform) {
         for (uint32_t i = 0; i < z_table_n; i++) {
             if (z_table[i].levels == c->levels &&
@@ -703,14 +828,14 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
         c->cur = 0;
     }
 
-    /* source tables */
+    /* source tables, sorted by cell; stable, so sources at one cell keep
+     * the reference's list order */
     {
         uint64_t total = 0;
         for (uint32_t r = 0; r < num_runs; r++) {
             total += (uint64_t)runs[r].num_sources * c->n_t;
         }
         RC(dalloc(c, (void**)&c->src_data, total * sizeof(double)));
-        std::vector<dev_source> srcs;
         uint64_t off = 0;
         for (uint32_t r = 0; r < num_runs; r++) {
             const mbh_problem& p = runs[r];
@@ -723,32 +848,15 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
                 s.gx = c->runs[r].base + p.sources[k].x_idx;
                 s.data = c->src_data + off;
                 s.type = p.sources[k].type;
-                srcs.push_back(s);
+                c->srcs.push_back(s);
                 off += c->n_t;
             }
         }
-        /* stable: sources at one cell keep the reference's list order */
-        std::stable_sort(srcs.begin(), srcs.end(),
+        std::stable_sort(c->srcs.begin(), c->srcs.end(),
                          [](const dev_source& a, const dev_source& b) {
                              return a.gx < b.gx;
                          });
-        std::vector<uint32_t> ts(2 * c->num_tiles);
-        for (uint64_t t = 0; t < c->num_tiles; t++) {
-            const int64_t lo = (int64_t)(t * c->owned) - c->halo;
-            const int64_t hi = lo + 64 * (int64_t)c->C;
-            auto a = std::lower_bound(srcs.begin(), srcs.end(), lo,
-                                      [](const dev_source& s, int64_t v) {
-                                          return (int64_t)s.gx < v;
-                                      });
-            auto b = std::lower_bound(srcs.begin(), srcs.end(), hi,
-                                      [](const dev_source& s, int64_t v) {
-                                          return (int64_t)s.gx < v;
-                                      });
-            ts[2 * t] = (uint32_t)(a - srcs.begin());
-            ts[2 * t + 1] = (uint32_t)(b - srcs.begin());
-        }
-        RC(upload(c, &c->src, srcs.data(), srcs.size()));
-        RC(upload(c, &c->tile_src, ts.data(), ts.size()));
+        RC(upload(c, &c->src, c->srcs.data(), c->srcs.size()));
     }
 
     /* record scratch and descriptors */
@@ -758,8 +866,6 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
         HIP_TRY(c, hipMemset(c->scratch, 0,
                              std::max<uint64_t>(1, scratch_total) *
                                  sizeof(double)));
-        std::vector<dev_record> recs;
-        std::vector<std::pair<uint64_t, uint32_t>> tile_rec;
         for (uint32_t r = 0; r < num_runs; r++) {
             const mbh_problem& p = runs[r];
             const run_layout& L = c->runs[r];
@@ -774,36 +880,19 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
                 d.type = R.type;
                 d.row_idx = R.row_idx;
                 d.col_idx = R.col_idx;
-                const uint32_t id = (uint32_t)recs.size();
-                recs.push_back(d);
-                const uint64_t t0 = d.gbegin / c->owned;
-                const uint64_t t1 = (d.gbegin + d.cols - 1) / c->owned;
-                for (uint64_t t = t0; t <= t1; t++) {
-                    tile_rec.emplace_back(t, id);
-                }
+                c->recs.push_back(d);
             }
         }
-        std::stable_sort(tile_rec.begin(), tile_rec.end(),
-                         [](const std::pair<uint64_t, uint32_t>& a,
-                            const std::pair<uint64_t, uint32_t>& b) {
-                             return a.first < b.first;
-                         });
-        std::vector<uint32_t> ptr(c->num_tiles + 1, 0), idx;
-        idx.reserve(tile_rec.size());
-        for (auto& tr : tile_rec) {
-            ptr[tr.first + 1]++;
-            idx.push_back(tr.second);
-        }
-        for (uint64_t t = 0; t < c->num_tiles; t++) {
-            ptr[t + 1] += ptr[t];
-        }
-        RC(upload(c, &c->rec, recs.data(), recs.size()));
-        RC(upload(c, &c->tile_rec_ptr, ptr.data(), ptr.size()));
-        RC(upload(c, &c->tile_rec_idx, idx.data(), idx.size()));
+        RC(upload(c, &c->rec, c->recs.data(), c->recs.size()));
     }
 
     RC(dalloc(c, (void**)&c->n_base, sizeof(uint64_t)));
     HIP_TRY(c, hipMemset(c->n_base, 0, sizeof(uint64_t)));
+    HIP_TRY(c, hipDeviceSynchronize());
+
+    /* launch geometry: measured, then the final per-tile tables */
+    RC(autotune(c, opt.temporal_block, opt.tile_cells));
+    RC(build_tile_tables(c, true));
     HIP_TRY(c, hipDeviceSynchronize());
 #undef RC
     *out = ctx.release();

@@ -20,10 +20,16 @@
 
 namespace mbh {
 
+/* Material constants are read through the constant address space: with a
+ * wave-uniform address these become scalar (s_load) loads even though the
+ * kernel stores to global memory. */
+typedef const __attribute__((address_space(4))) double* cdp;
+
 /* ---- complex helpers (naive formulas, the libstdc++ fast path) --------- */
 struct cplx {
     double re, im;
 };
+typedef const __attribute__((address_space(4))) cplx* ccp;
 
 __device__ __forceinline__ cplx cmul(cplx a, cplx b)
 {
@@ -164,12 +170,14 @@ struct algo_noop {
     struct state {
         double v[STATE];
     };
-    __device__ static void update(const double*, state&, double, double&) {}
-    __device__ static double inversion(const double*, const state&)
+    __device__ static bool has_qm(cdp) { return false; }
+    __device__ static void update(cdp, state&, double, double&) {}
+    template<int C>
+    __device__ static void update_multi(cdp, state*, const double*, double*)
     {
-        return 0.0;
     }
-    __device__ static double population(const double*, const state&, int)
+    __device__ static double inversion(cdp, const state&) { return 0.0; }
+    __device__ static double population(cdp, const state&, int)
     {
         return 0.0;
     }
@@ -199,10 +207,10 @@ struct algo_cayley {
 
     /* propagate_dissipation (algo_lindblad_reg_cayley.hpp:206-217):
      * coherences *= D2 (elementwise), diagonal <- D1 * diagonal. */
-    __device__ static void dissipate(const double* q, const cplx* d, cplx* o)
+    __device__ static void dissipate(cdp q, const cplx* d, cplx* o)
     {
-        const double* D1 = q + OFF_D1;
-        const double* D2 = q + OFF_D2;
+        cdp D1 = q + OFF_D1;
+        cdp D2 = q + OFF_D2;
 #pragma unroll
         for (int i = 0; i < N * N; i++) {
             o[i] = cplx{ D2[i] * d[i].re, D2[i] * d[i].im };
@@ -219,14 +227,13 @@ struct algo_cayley {
         }
     }
 
-    __device__ static void update(const double* q, state& s, double e,
-                                  double& p)
+    __device__ static bool has_qm(cdp q) { return q[0] != 0.0; }
+
+    /* branch-free body; callers check has_qm */
+    __device__ static void update(cdp q, state& s, double e, double& p)
     {
-        if (q[0] == 0.0) {
-            return;
-        }
-        const cplx* H0 = reinterpret_cast<const cplx*>(q + OFF_H0);
-        const cplx* mu = reinterpret_cast<const cplx*>(q + OFF_MU);
+        ccp H0 = (ccp)(q + OFF_H0);
+        ccp mu = (ccp)(q + OFF_MU);
         const double prop_im = q[3];
         const double rhs_im = q[4];
         cplx* d = reinterpret_cast<cplx*>(s.v);
@@ -243,9 +250,13 @@ struct algo_cayley {
             B[i] = cplx{ di + 0.5 * m.re, 0.5 * m.im };
         }
         /* Cayley propagator U2 = A^-1 B */
-        cplx Ai[N * N], U2[N * N];
+        cplx Ai[N * N], U2[N * N], Hm[N * N];
         cinverse<N>(A, Ai);
         cmatmul<N>(Ai, B, U2);
+#pragma unroll
+        for (int i = 0; i < N * N; i++) {
+            Hm[i] = cplx{ H0[i].re, H0[i].im };
+        }
 
         /* dissipation half step, unitary full step, dissipation half step */
         cplx d1[N * N], t[N * N];
@@ -256,11 +267,11 @@ struct algo_cayley {
 
         /* polarization derivative: rhs = -i/hbar [H0, d] + D(d);
          * p = n Re tr(rhs mu) */
-        const double* r1 = q + OFF_d1;
-        const double* r2 = q + OFF_d2;
+        cdp r1 = q + OFF_d1;
+        cdp r2 = q + OFF_d2;
         cplx c1[N * N], c2[N * N];
-        cmatmul<N>(H0, d, c1);
-        cmatmul<N>(d, H0, c2);
+        cmatmul<N>(Hm, d, c1);
+        cmatmul<N>(d, Hm, c2);
         cplx rhs[N * N];
 #pragma unroll
         for (int i = 0; i < N * N; i++) {
@@ -294,12 +305,22 @@ struct algo_cayley {
         p = q[1] * tr;
     }
 
+    template<int C>
+    __device__ static void update_multi(cdp q, state* s, const double* e,
+                                        double* p)
+    {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            update(q, s[c], e[c], p[c]);
+        }
+    }
+
     /* calc_inversion / calc_population (algo_lindblad_reg_cayley.hpp:259-267) */
-    __device__ static double inversion(const double*, const state& s)
+    __device__ static double inversion(cdp, const state& s)
     {
         return s.v[2 * (N + 1)] - s.v[0];
     }
-    __device__ static double population(const double*, const state& s, int m)
+    __device__ static double population(cdp, const state& s, int m)
     {
         double r = 0.0;
 #pragma unroll
@@ -377,19 +398,18 @@ struct algo_rodr {
     static constexpr int OFF_C1 = OFF_TH + B;
     static constexpr int OFF_C2 = OFF_C1 + B * K * K;
 
-    __device__ static void update(const double* q, state& s, double e,
-                                  double& p)
+    __device__ static bool has_qm(cdp q) { return q[0] != 0.0; }
+
+    /* callers check has_qm */
+    __device__ static void update(cdp q, state& s, double e, double& p)
     {
-        if (q[0] == 0.0) {
-            return;
-        }
         const double M_CP = q[1];
         const double d_t = q[2];
-        const double* M = q + OFF_M;
-        const double* A1 = q + OFF_A1;
-        const double* v = q + OFF_V;
-        const double* d_eq = q + OFF_DEQ;
-        const double* d_in = q + OFF_DIN;
+        cdp M = q + OFF_M;
+        cdp A1 = q + OFF_A1;
+        cdp v = q + OFF_V;
+        cdp d_eq = q + OFF_DEQ;
+        cdp d_in = q + OFF_DIN;
         double* d = s.v;
 
         /* d1 = A1 (d + d_in) - d_in */
@@ -411,8 +431,8 @@ struct algo_rodr {
         /* d2 = A2 d1 */
         double d2[K];
         if constexpr (N == 2) {
-            const double* U = q + OFF_U;
-            const double* U2 = q + OFF_U2;
+            cdp U = q + OFF_U;
+            cdp U2 = q + OFF_U2;
             const double th1 = q[3];
             double x = th1 * e * d_t;
             double sa = sin(x) / th1;
@@ -429,21 +449,19 @@ struct algo_rodr {
                 d2[i] = acc;
             }
         } else {
-            const int nth = (int)q[4];
-            const double* th = q + OFF_TH;
+            /* planes b >= n_theta have theta = 0 and zero C1, C2 in the
+             * blob: their terms are exact zeros and leave A2 unchanged, so
+             * every plane is evaluated without a data-dependent branch */
+            cdp th = q + OFF_TH;
             double sv[B], vv[B];
 #pragma unroll
             for (int b = 0; b < B; b++) {
-                sv[b] = 0.0;
-                vv[b] = 0.0;
-                if (b < nth) {
-                    double x = th[b] * e * d_t;
-                    sv[b] = sin(x);
-                    vv[b] = 1 - cos(x);
-                }
+                double x = th[b] * e * d_t;
+                sv[b] = sin(x);
+                vv[b] = 1 - cos(x);
             }
-            const double* C1 = q + OFF_C1;
-            const double* C2 = q + OFF_C2;
+            cdp C1 = q + OFF_C1;
+            cdp C2 = q + OFF_C2;
             /* rows: unrolled for small K; for large K one row per iteration
              * with the result routed through a compare-select chain so that
              * no register array is indexed dynamically */
@@ -456,11 +474,9 @@ struct algo_rodr {
                     double a2 = (i == k) ? 1.0 : 0.0;
 #pragma unroll
                     for (int b = 0; b < B; b++) {
-                        if (b < nth) {
-                            a2 = a2 +
-                                (sv[b] * C1[(b * K + i) * K + k] +
-                                 vv[b] * C2[(b * K + i) * K + k]);
-                        }
+                        a2 = a2 +
+                            (sv[b] * C1[(b * K + i) * K + k] +
+                             vv[b] * C2[(b * K + i) * K + k]);
                     }
                     acc = (k == 0) ? a2 * d1[0] : acc + a2 * d1[k];
                 }
@@ -504,44 +520,57 @@ struct algo_rodr {
         p = pacc;
     }
 
+    template<int C>
+    __device__ static void update_multi(cdp q, state* s, const double* e,
+                                        double* p)
+    {
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            update(q, s[c], e[c], p[c]);
+        }
+    }
+
     /* calc_inversion (algo_lindblad_cvr_rodr.hpp:274-277) */
-    __device__ static double inversion(const double*, const state& s)
+    __device__ static double inversion(cdp, const state& s)
     {
         return s.v[N * (N - 1)];
     }
-    __device__ static double population(const double*, const state& s, int m)
+    __device__ static double population(cdp, const state& s, int m)
     {
         return cv_population<N>(s.v, m);
     }
 };
 
 /* ---- sin/cos for the rotated-basis form --------------------------------
- * Taylor series of sin and cos about 0, evaluated in Horner form in x^2,
- * for |x| <= 0.5 (truncation error below 3e-20 relative); larger
- * arguments use the library sincos. */
+ * Taylor series of sin (to x^13) and cos (to x^14) about 0 in Horner form
+ * in x^2; for |x| <= 0.25 the truncation error is below 1e-21 relative. */
+constexpr double TAYLOR_MAX = 0.25;
+
+__device__ __forceinline__ void taylor_sincos(double x, double& s, double& c)
+{
+    const double x2 = x * x;
+    double ps = 1.6059043836821614599e-10;        /* 1/13! */
+    ps = fma(ps, x2, -2.5052108385441718775e-08); /* -1/11! */
+    ps = fma(ps, x2, 2.7557319223985890653e-06);  /* 1/9! */
+    ps = fma(ps, x2, -1.9841269841269841270e-04); /* -1/7! */
+    ps = fma(ps, x2, 8.3333333333333333333e-03);  /* 1/5! */
+    ps = fma(ps, x2, -1.6666666666666666667e-01); /* -1/3! */
+    s = fma(ps * x2, x, x);
+    double pc = -1.1470745597729724714e-11;       /* -1/14! */
+    pc = fma(pc, x2, 2.0876756987868098979e-09);  /* 1/12! */
+    pc = fma(pc, x2, -2.7557319223985890653e-07); /* -1/10! */
+    pc = fma(pc, x2, 2.4801587301587301587e-05);  /* 1/8! */
+    pc = fma(pc, x2, -1.3888888888888888889e-03); /* -1/6! */
+    pc = fma(pc, x2, 4.1666666666666666667e-02);  /* 1/4! */
+    pc = fma(pc, x2, -0.5);                       /* -1/2! */
+    c = fma(pc, x2, 1.0);
+}
+
+/* Taylor series for |x| <= TAYLOR_MAX, library sincos otherwise. */
 __device__ __forceinline__ void zsincos(double x, double& s, double& c)
 {
-    if (fabs(x) <= 0.5) {
-        double x2 = x * x;
-        /* sin: x (1 - x^2/3! + x^4/5! - ... - x^14/15!) */
-        double ps = -7.6471637318198164759e-13; /* -1/15! */
-        ps = fma(ps, x2, 1.6059043836821614599e-10);  /* 1/13! */
-        ps = fma(ps, x2, -2.5052108385441718775e-08); /* -1/11! */
-        ps = fma(ps, x2, 2.7557319223985890653e-06);  /* 1/9! */
-        ps = fma(ps, x2, -1.9841269841269841270e-04); /* -1/7! */
-        ps = fma(ps, x2, 8.3333333333333333333e-03);  /* 1/5! */
-        ps = fma(ps, x2, -1.6666666666666666667e-01); /* -1/3! */
-        s = fma(ps * x2, x, x);
-        /* cos: 1 - x^2/2! + x^4/4! - ... + x^16/16! */
-        double pc = 4.7794773323873852974e-14;         /* 1/16! */
-        pc = fma(pc, x2, -1.1470745597729724714e-11);  /* -1/14! */
-        pc = fma(pc, x2, 2.0876756987868098979e-09);   /* 1/12! */
-        pc = fma(pc, x2, -2.7557319223985890653e-07);  /* -1/10! */
-        pc = fma(pc, x2, 2.4801587301587301587e-05);   /* 1/8! */
-        pc = fma(pc, x2, -1.3888888888888888889e-03);  /* -1/6! */
-        pc = fma(pc, x2, 4.1666666666666666667e-02);   /* 1/4! */
-        pc = fma(pc, x2, -0.5);                        /* -1/2! */
-        c = fma(pc, x2, 1.0);
+    if (fabs(x) <= TAYLOR_MAX) {
+        taylor_sincos(x, s, c);
     } else {
         sincos(x, &s, &c);
     }
@@ -572,59 +601,116 @@ struct algo_zform {
     static constexpr int OFF_QTA1 = OFF_QT + K * K;
     static constexpr int OFF_DIN = OFF_QTA1 + K * K;
 
-    __device__ static void update(const double* q, state& s, double e,
-                                  double& p)
+    __device__ static bool has_qm(cdp q) { return q[0] != 0.0; }
+
+    /* C cells of one material: straight-line code with the cell index
+     * innermost, so the independent chains interleave.  Planes beyond
+     * n_theta have angle 0 (identity rotation, exact). */
+    template<int C>
+    __device__ static void update_multi(cdp q, state* s, const double* e,
+                                        double* p)
     {
-        if (q[0] == 0.0) {
-            return;
+        cdp th = q + OFF_TH;
+        cdp W = q + OFF_W;
+        cdp cv = q + OFF_C;
+        cdp a = q + OFF_A;
+
+        double sn[C][B], cs[C][B];
+        bool small = true;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                const double x = th[b] * e[c];
+                sn[c][b] = x;
+                small = small && (fabs(x) <= TAYLOR_MAX);
+            }
         }
-        const int nth = (int)q[1];
-        const double* th = q + OFF_TH;
-        const double* W = q + OFF_W;
-        const double* c = q + OFF_C;
-        const double* a = q + OFF_A;
-        double* y = s.v;
+        if (__all(small)) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    taylor_sincos(sn[c][b], sn[c][b], cs[c][b]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    zsincos(sn[c][b], sn[c][b], cs[c][b]);
+                }
+            }
+        }
 
         /* r = R(e) y */
-        double r[K];
+        double r[C][K];
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            r[k] = y[k];
-        }
+        for (int c = 0; c < C; c++) {
 #pragma unroll
-        for (int b = 0; b < B; b++) {
-            if (b < nth) {
-                double sn, cs;
-                zsincos(th[b] * e, sn, cs);
-                double y0 = y[2 * b], y1 = y[2 * b + 1];
-                r[2 * b] = cs * y0 - sn * y1;
-                r[2 * b + 1] = fma(sn, y0, cs * y1);
+            for (int k = 2 * B; k < K; k++) {
+                r[c][k] = s[c].v[k];
+            }
+#pragma unroll
+            for (int b = 0; b < B; b++) {
+                const double y0 = s[c].v[2 * b], y1 = s[c].v[2 * b + 1];
+                r[c][2 * b] = fma(cs[c][b], y0, -(sn[c][b] * y1));
+                r[c][2 * b + 1] = fma(sn[c][b], y0, cs[c][b] * y1);
             }
         }
         /* p = a . r + b */
-        double pacc = q[2];
+        double pa[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            pa[c] = q[2];
+        }
 #pragma unroll
         for (int k = 0; k < K; k++) {
-            pacc = fma(a[k], r[k], pacc);
+            const double ak = a[k];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                pa[c] = fma(ak, r[c][k], pa[c]);
+            }
         }
-        p = pacc;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            p[c] = pa[c];
+        }
         /* y' = W r + c */
 #pragma unroll
         for (int i = 0; i < K; i++) {
-            double acc = c[i];
+            double acc[C];
+            const double ci = cv[i];
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                acc[c] = ci;
+            }
 #pragma unroll
             for (int k = 0; k < K; k++) {
-                acc = fma(W[i * K + k], r[k], acc);
+                const double w = W[i * K + k];
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    acc[c] = fma(w, r[c][k], acc[c]);
+                }
             }
-            y[i] = acc;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                s[c].v[i] = acc[c];
+            }
         }
     }
 
-    /* component k of the reference coherence vector after the step */
-    __device__ static double dcomp(const double* q, const state& s, int k)
+    __device__ static void update(cdp q, state& s, double e, double& p)
     {
-        const double* P = q + OFF_P;
-        const double* o = q + OFF_O;
+        update_multi<1>(q, &s, &e, &p);
+    }
+
+    /* component k of the reference coherence vector after the step */
+    __device__ static double dcomp(cdp q, const state& s, int k)
+    {
+        cdp P = q + OFF_P;
+        cdp o = q + OFF_O;
         double acc = o[k];
 #pragma unroll
         for (int j = 0; j < K; j++) {
@@ -632,15 +718,14 @@ struct algo_zform {
         }
         return acc;
     }
-    __device__ static double inversion(const double* q, const state& s)
+    __device__ static double inversion(cdp q, const state& s)
     {
         if (q[0] == 0.0) {
             return 0.0;
         }
         return dcomp(q, s, N * (N - 1));
     }
-    __device__ static double population(const double* q, const state& s,
-                                        int m)
+    __device__ static double population(cdp q, const state& s, int m)
     {
         double d[K];
 #pragma unroll

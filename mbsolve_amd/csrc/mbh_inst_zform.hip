@@ -22,11 +22,11 @@ __global__ __launch_bounds__(64) void zform_to_y(ZArgs a)
         y[j] = 0.0;
     }
     for_each_material(m, [&](int32_t mm) {
-        const double* q = a.zq + (uint64_t)mm * a.zstride;
+        cdp q = (cdp)a.zq + (uint64_t)mm * a.zstride;
         if (q[0] != 0.0) {
-            const double* Qt = q + algo_zform<N>::OFF_QT;
-            const double* QtA1 = q + algo_zform<N>::OFF_QTA1;
-            const double* din = q + algo_zform<N>::OFF_DIN;
+            cdp Qt = q + algo_zform<N>::OFF_QT;
+            cdp QtA1 = q + algo_zform<N>::OFF_QTA1;
+            cdp din = q + algo_zform<N>::OFF_DIN;
             double t[K];
 #pragma unroll
             for (int j = 0; j < K; j++) {
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(64) void zform_from_y(ZArgs a)
         d[j] = 0.0;
     }
     for_each_material(m, [&](int32_t mm) {
-        const double* q = a.zq + (uint64_t)mm * a.zstride;
+        cdp q = (cdp)a.zq + (uint64_t)mm * a.zstride;
         if (q[0] != 0.0) {
 #pragma unroll
             for (int j = 0; j < K; j++) {

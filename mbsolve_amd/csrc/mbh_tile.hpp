@@ -58,6 +58,8 @@ struct KArgs {
     void* p_out;
     void* d_out;
     uint64_t G;       /* cells in the super-grid */
+    uint64_t Gown;    /* cells owned by tiles: G minus the trailing pseudo
+                         cell, which always holds zeros */
     uint64_t dstride; /* elements between density components */
     const int32_t* mat;   /* [G], -1: pseudo cell */
     const uint8_t* hzero; /* [G] */
@@ -88,6 +90,35 @@ __device__ __forceinline__ void st(void* p, uint64_t i, double v)
     static_cast<ST*>(p)[i] = (ST)v;
 }
 
+/* read-only tables through the constant address space (scalar loads for
+ * uniform indices) */
+template<typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T*
+as_const(const T* p)
+{
+    return (const __attribute__((address_space(4))) T*)p;
+}
+
+/* Whole-wavefront neighbour exchange with DPP: wave_shl:1 gives lane i the
+ * value of lane i+1, wave_shr:1 the value of lane i-1 (the edge lane gets
+ * 0; edge cells are halo cells).  Register-to-register, no LDS round trip. */
+__device__ __forceinline__ double from_right(double x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x130,
+                                               0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x130,
+                                               0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_left(double x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x138,
+                                               0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x138,
+                                               0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
 /* Runs `fn(material)` for every lane whose `m >= 0`, one distinct material
  * at a time; inside fn the material index is wave-uniform (SGPR), so
  * constant reads through it become scalar loads. */
@@ -109,14 +140,42 @@ __device__ __forceinline__ void for_each_material(int32_t m, F&& fn)
     }
 }
 
+__device__ __forceinline__ dev_record load_record(const dev_record* base,
+                                                 uint32_t i)
+{
+    const auto* r = as_const(base) + i;
+    dev_record R;
+    R.gbegin = r->gbegin;
+    R.cols = r->cols;
+    R.rows = r->rows;
+    R.ratio = r->ratio;
+    R.dst = r->dst;
+    R.type = r->type;
+    R.row_idx = r->row_idx;
+    R.col_idx = r->col_idx;
+    R.pad = 0;
+    return R;
+}
+
+/* copy_list_entry_dev::hasto_record (copy_list_entry.hpp:74-80) plus the
+ * row bound */
+__device__ __forceinline__ bool record_now(double ratio, uint64_t rows,
+                                           uint64_t n)
+{
+    bool now = (n == 0);
+    if (!now) {
+        now = floor(ratio * (double)(n - 1)) != floor(ratio * (double)n);
+    }
+    return now && (uint64_t)floor(ratio * (double)n) < rows;
+}
+
 template<class A>
 __device__ __forceinline__ double record_value(const KArgs& k, uint32_t type,
-                                               uint32_t row, uint32_t col,
-                                               int32_t m, double e, double h,
-                                               double p,
+                                               uint32_t row, int32_t m,
+                                               double e, double h, double p,
                                                const typename A::state& d)
 {
-    const double* q = k.qm + (uint64_t)(m < 0 ? 0 : m) * k.qstride;
+    cdp q = (cdp)k.qm + (uint64_t)(m < 0 ? 0 : m) * k.qstride;
     switch (type) {
     case MBH_REC_ELECTRIC:
         return e;
@@ -131,17 +190,24 @@ __device__ __forceinline__ double record_value(const KArgs& k, uint32_t type,
     }
 }
 
+/* sources staged per launch in LDS (values for the block's steps) */
+constexpr int TILE_SOURCES = 8;
+constexpr int MAX_BLOCK = 64; /* largest temporal block */
+
 template<class A, int C, typename ST>
 __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
 {
+    __shared__ double s_src[TILE_SOURCES][MAX_BLOCK];
+
     const int lane = threadIdx.x;
     const uint64_t t = blockIdx.x;
     const int64_t own_lo = (int64_t)(t * k.owned);
     const int64_t own_hi =
-        ((own_lo + (int64_t)k.owned) < (int64_t)k.G) ? own_lo + k.owned
-                                                      : (int64_t)k.G;
+        ((own_lo + (int64_t)k.owned) < (int64_t)k.Gown) ? own_lo + k.owned
+                                                         : (int64_t)k.Gown;
     const int64_t gl = own_lo - (int64_t)k.halo + (int64_t)lane * C;
-    const uint64_t n0 = *k.n_base + k.n_off;
+    const uint64_t n0 = *as_const(k.n_base) + k.n_off;
+    const uint32_t nsteps = k.nsteps;
 
     double e[C], h[C], p[C];
     double fa[C], fb[C], fc[C], fg[C], fx[C];
@@ -174,16 +240,48 @@ __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
         }
     }
 
-    const uint32_t s_lo = k.tile_src[2 * t];
-    const uint32_t s_hi = k.tile_src[2 * t + 1];
-    const uint32_t r_lo = k.tile_rec_ptr[t];
-    const uint32_t r_hi = k.tile_rec_ptr[t + 1];
+    /* material: one uniform material for the whole tile (the common case)
+     * is decided once per launch */
+    const int32_t m0 = __builtin_amdgcn_readfirstlane(m[0]);
+    bool same = true;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        same = same && (m[c] == m0);
+    }
+    const bool uniform = __all(same);
+    cdp q0 = (cdp)k.qm + (uint64_t)(m0 < 0 ? 0 : m0) * k.qstride;
+    const bool q0_active = uniform && m0 >= 0 && A::DC > 0 && A::has_qm(q0);
 
-    for (uint32_t s = 0; s < k.nsteps; s++) {
+    /* sources of this tile: their values for the block's steps go to LDS */
+    const uint32_t s_lo = as_const(k.tile_src)[2 * t];
+    const uint32_t s_hi = as_const(k.tile_src)[2 * t + 1];
+    const uint32_t nsrc = s_hi - s_lo;
+    const bool src_staged = nsrc <= TILE_SOURCES && nsteps <= MAX_BLOCK;
+    if (src_staged && nsrc > 0) {
+        for (uint32_t j = 0; j < nsrc; j++) {
+            const double* data = as_const(k.src)[s_lo + j].data;
+            s_src[j][lane] = (lane < (int)nsteps) ? data[n0 + lane] : 0.0;
+        }
+        __syncthreads();
+    }
+
+    /* steps of this block at which any of the tile's records is due */
+    const uint32_t r_lo = as_const(k.tile_rec_ptr)[t];
+    const uint32_t r_hi = as_const(k.tile_rec_ptr)[t + 1];
+    uint64_t rec_steps = 0;
+    for (uint32_t ri = r_lo; ri < r_hi; ri++) {
+        const uint32_t idx = as_const(k.tile_rec_idx)[ri];
+        const double ratio = as_const(k.rec)[idx].ratio;
+        const uint64_t rows = as_const(k.rec)[idx].rows;
+        rec_steps |= __ballot(lane < (int)nsteps &&
+                              record_now(ratio, rows, n0 + lane));
+    }
+
+    for (uint32_t s = 0; s < nsteps; s++) {
         const uint64_t n = n0 + s;
 
         /* E update (solver_cpu_fdtd.tpp:238-243) */
-        const double h_right = __shfl_down(h[0], 1);
+        const double h_right = from_right(h[0]);
         double en[C];
 #pragma unroll
         for (int c = 0; c < C; c++) {
@@ -194,20 +292,25 @@ __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
         }
 
         /* sources, in list order (solver_cpu_fdtd.tpp:247-255) */
-        for (uint32_t si = s_lo; si < s_hi; si++) {
-            const dev_source S = k.src[si];
-            const int64_t off = (int64_t)S.gx - gl;
-            const double v = S.data[n];
+        if (nsrc > 0) {
+            for (uint32_t j = 0; j < nsrc; j++) {
+                const uint64_t gx = as_const(k.src)[s_lo + j].gx;
+                const uint32_t type = as_const(k.src)[s_lo + j].type;
+                const double v = src_staged
+                    ? s_src[j][s]
+                    : as_const(k.src)[s_lo + j].data[n];
+                const int64_t off = (int64_t)gx - gl;
 #pragma unroll
-            for (int c = 0; c < C; c++) {
-                if (off == c) {
-                    en[c] = (S.type == MBH_SRC_HARD) ? v : en[c] + v;
+                for (int c = 0; c < C; c++) {
+                    if (off == c) {
+                        en[c] = (type == MBH_SRC_HARD) ? v : en[c] + v;
+                    }
                 }
             }
         }
 
         /* H update and boundary condition (solver_cpu_fdtd.tpp:263-275) */
-        const double e_left = __shfl_up(en[C - 1], 1);
+        const double e_left = from_left(en[C - 1]);
 #pragma unroll
         for (int c = 0; c < C; c++) {
             const double ep = (c > 0) ? en[(c + C - 1) % C] : e_left;
@@ -218,40 +321,45 @@ __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
 #pragma unroll
         for (int c = 0; c < C; c++) {
             e[c] = en[c];
-            if constexpr (A::DC > 0) {
-                for_each_material(m[c], [&](int32_t mm) {
-                    A::update(k.qm + (uint64_t)mm * k.qstride, d[c], e[c],
-                              p[c]);
-                });
+        }
+        if constexpr (A::DC > 0) {
+            if (q0_active) {
+                A::template update_multi<C>(q0, d, e, p);
+            } else if (!uniform) {
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    for_each_material(m[c], [&](int32_t mm) {
+                        cdp q = (cdp)k.qm + (uint64_t)mm * k.qstride;
+                        if (A::has_qm(q)) {
+                            A::update(q, d[c], e[c], p[c]);
+                        }
+                    });
+                }
             }
         }
 
         /* record copy (solver_cpu_fdtd.tpp:288-325) */
-        for (uint32_t ri = r_lo; ri < r_hi; ri++) {
-            const dev_record R = k.rec[k.tile_rec_idx[ri]];
-            bool now = (n == 0);
-            if (!now) {
-                now = floor(R.ratio * (double)(n - 1)) !=
-                    floor(R.ratio * (double)n);
-            }
-            if (!now) {
-                continue;
-            }
-            const uint64_t row = (uint64_t)floor(R.ratio * (double)n);
-            if (row >= R.rows) {
-                continue;
-            }
-            if (R.type == MBH_REC_DENSITY && R.row_idx != R.col_idx) {
-                continue; /* coherences are not recorded */
-            }
+        if ((rec_steps >> s) & 1ull) {
+            for (uint32_t ri = r_lo; ri < r_hi; ri++) {
+                const dev_record R =
+                    load_record(k.rec, as_const(k.tile_rec_idx)[ri]);
+                if (!record_now(R.ratio, R.rows, n)) {
+                    continue;
+                }
+                if (R.type == MBH_REC_DENSITY && R.row_idx != R.col_idx) {
+                    continue; /* coherences are not recorded */
+                }
+                const uint64_t row = (uint64_t)floor(R.ratio * (double)n);
 #pragma unroll
-            for (int c = 0; c < C; c++) {
-                const int64_t g = gl + c;
-                if (g >= own_lo && g < own_hi && g >= (int64_t)R.gbegin &&
-                    g < (int64_t)(R.gbegin + R.cols)) {
-                    R.dst[row * R.cols + (uint64_t)(g - (int64_t)R.gbegin)] =
-                        record_value<A>(k, R.type, R.row_idx, R.col_idx, m[c],
-                                        e[c], h[c], p[c], d[c]);
+                for (int c = 0; c < C; c++) {
+                    const int64_t g = gl + c;
+                    if (g >= own_lo && g < own_hi &&
+                        g >= (int64_t)R.gbegin &&
+                        g < (int64_t)(R.gbegin + R.cols)) {
+                        R.dst[row * R.cols + (uint64_t)(g - (int64_t)R.gbegin)] =
+                            record_value<A>(k, R.type, R.row_idx, m[c], e[c],
+                                            h[c], p[c], d[c]);
+                    }
                 }
             }
         }

@@ -441,8 +441,9 @@ def build_problem(spec):
             rec.get("position", -1.0)
         rtype = rec.get("type", {"e": 0, "p": 1, "h": 2, "d": 3,
                                  "i": 4}[name[0]])
-        row = rec.get("row", 1) - 1 if "row" in rec else 0
-        col = rec.get("col", 1) - 1 if "col" in rec else 0
+        # one-based indices, unsigned as in record.cpp:58-69 (0 wraps)
+        row = (rec["row"] - 1) & 0xFFFFFFFF if "row" in rec else 0
+        col = (rec["col"] - 1) & 0xFFFFFFFF if "col" in rec else 0
         if interval <= dt:
             rows, ratio = n_t, 1.0
         else:
