@@ -5,10 +5,12 @@ Workload (default): the three-level V system on the Rodrigues propagator,
 .v3_rodr).  A "step" is one FDTD time step of the whole grid: E update,
 sources, H update, density update of every cell, record copy.
 
-The timed form is the mixed one (FP32 state storage, FP64 arithmetic,
-rotated-basis Rodrigues); the plain FP64 form (reference operation order) is
-timed on the same steps, and the records of both are compared (relative error
-max|x - x_fp64| / max|x_fp64|, per record).  Device time comes from HIP events
+Candidate fast forms (default: "mixed" = FP32 state storage with FP64
+arithmetic and rotated-basis Rodrigues, "zform" = the same with FP64 storage)
+and the plain FP64 form (reference operation order) are timed on the same
+steps; the records of each fast form are compared with the FP64 form
+(relative error max|x - x_fp64| / max|x_fp64|, per record) and the headline
+value is the fastest form within the 1e-5 bound.  Device time comes from HIP events
 on the solver's stream; the wall clock brackets the same region after a
 device synchronisation.  A bounded sample of the CPU FP64 oracle (the
 restatement of the reference's cpu-fdtd solver) is timed on the host cores.
@@ -126,6 +128,56 @@ def cpu_baseline(specs, form_name, seconds=15.0):
                       "%.1f s" % (nx, nsteps, probe, el)}
 
 
+def measure(specs, form, warmup, steps, opts):
+    """Builds the solver of `form`, runs warm-up + timed steps, returns
+    (value cu/s, device ms, wall s, info, records)."""
+    sol = build_solver(specs, form, **opts)
+    prob0 = sol.problem() if not hasattr(sol, "num_runs") else sol.problem(0)
+    n_t = int(prob0["num_timesteps"])
+    warmup = min(warmup, n_t - 1)
+    steps = min(steps if steps else n_t - warmup, n_t - warmup)
+    dev_ms, wall = timed_run(sol, warmup, steps)
+    info = sol.info()
+    cells = int(info["total_cells"])
+    recs = results_of(sol)
+    del sol
+    return {"value": cells * steps / (dev_ms * 1e-3), "dev_ms": dev_ms,
+            "wall": wall, "info": info, "records": recs, "steps": steps,
+            "warmup": warmup, "n_t": n_t, "cells": cells}
+
+
+def rel_errs(recs, ref):
+    errs = []
+    for a, b in zip(recs, ref):
+        scale = np.max(np.abs(b))
+        d = np.max(np.abs(a - b))
+        errs.append(float(d / scale) if scale > 0 else float(d))
+    return errs
+
+
+def copy_calibration(nbytes, reps=5):
+    """Device-to-device hipMemcpy of `nbytes` (the runtime's plain copy
+    kernel), for calibrating the FETCH_SIZE / WRITE_SIZE counters against a
+    known byte count."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or \
+            hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 0, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # device to device
+        hip.hipDeviceSynchronize()
+        el = time.perf_counter() - t0
+        log("copy %d bytes: %.3f ms, %.1f GB/s (read + write)"
+            % (nbytes, el * 1e3, 2 * nbytes / el / 1e9))
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,19 +185,28 @@ def main():
                     help="timed steps (default: the rest of the run)")
     ap.add_argument("--warmup", type=int, default=2048)
     ap.add_argument("--config", default="v3")
-    ap.add_argument("--form", default="mixed")
+    ap.add_argument("--forms", default="mixed,zform",
+                    help="candidate fast forms; the headline is the fastest "
+                         "one within the error bound of the FP64 form")
+    ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--temporal-block", type=int, default=0)
     ap.add_argument("--tile-cells", type=int, default=0)
     ap.add_argument("--graph-launches", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; runs on %d GPUs are "
-              "not supported (the sharded layout is described in DESIGN.md)"
-              % args.gpus, file=sys.stderr)
+              "not supported (each GPU would own a slab of the grid or a "
+              "share of the batched runs)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import mbsolve_amd.lib as mb
@@ -159,47 +220,54 @@ def main():
                 tile_cells=args.tile_cells,
                 graph_launches=args.graph_launches)
 
-    log("building", args.form, "solver for", workload)
-    sol = build_solver(specs, args.form, **opts)
-    prob0 = sol.problem() if not hasattr(sol, "num_runs") else sol.problem(0)
-    n_t = int(prob0["num_timesteps"])
-    warmup = min(args.warmup, n_t - 1)
-    steps = args.steps if args.steps else n_t - warmup
-    steps = min(steps, n_t - warmup)
-    info = sol.info()
-    cells = int(info["total_cells"])
-    log("info", info)
-    dev_ms, wall = timed_run(sol, warmup, steps)
-    info = sol.info()
-    launches = int(info["launches"])
-    cu = cells * steps
-    value = cu / (dev_ms * 1e-3)
-    log("%s: %.3f ms device, %.3f s wall, %d launches, %.4g cu/s"
-        % (args.form, dev_ms, wall, launches, value))
-    rec_mixed = results_of(sol)
-    del sol
+    forms = [f for f in args.forms.split(",") if f]
+    results = {}
+    for form in forms:
+        log("measuring form", form, "on", workload)
+        results[form] = measure(specs, form, args.warmup, args.steps, opts)
+        r = results[form]
+        log("  %s: %.3f ms device, %d launches, %.4g cu/s, info %s"
+            % (form, r["dev_ms"], r["info"]["launches"], r["value"],
+               r["info"]))
+    ref = None
+    if not args.no_fp64:
+        log("measuring the FP64 reference-order form")
+        ref = measure(specs, "", args.warmup, args.steps, opts)
+        log("  fp64: %.3f ms device, %.4g cu/s" % (ref["dev_ms"],
+                                                   ref["value"]))
+        for form, r in results.items():
+            r["errs"] = rel_errs(r["records"], ref["records"])
+            log("  %s max rel err vs fp64: %s" % (form, r["errs"]))
 
-    storage = "f32" if args.form == "mixed" else "f64"
-    per_launch_units = cells * info["temporal_block"]
-    launch_ms = dev_ms / max(launches, 1)
+    # headline: fastest form within the error bound (if measured)
+    ok = [f for f in forms
+          if ref is None or max(results[f]["errs"] or [0.0]) <= args.tol]
+    pool = ok if ok else forms
+    head = max(pool, key=lambda f: results[f]["value"])
+    r = results[head]
+    info = r["info"]
+    storage = "f32" if head == "mixed" else "f64"
+    launches = int(info["launches"])
+    per_launch_units = r["cells"] * info["temporal_block"]
+    launch_ms = r["dev_ms"] / max(launches, 1)
     bpu = ALGO_BYTES.get((args.config, storage))
     achieved = bpu * per_launch_units / (launch_ms * 1e-3) / 1e9
     fpu = ALGO_FLOPS.get(args.config)
     out = {
         "metric": "cell-updates/s, mixed vs FP64 GPU at <=1e-5 rel err; "
                   "% HBM/MFMA roofline",
-        "value": value, "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": steps, "warmup": warmup,
-        "ms_per_step": dev_ms / steps, "higher_is_better": True,
+        "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": r["steps"], "warmup": r["warmup"],
+        "ms_per_step": r["dev_ms"] / r["steps"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "storage": storage, "data": "synthetic",
-        "config": {"workload": workload, "cells": cells,
-                   "time_steps_total": n_t, "form": args.form,
+        "config": {"workload": workload, "cells": r["cells"],
+                   "time_steps_total": r["n_t"], "form": head,
                    "temporal_block": info["temporal_block"],
                    "tile_cells": info["tile_cells"],
                    "owned_cells": info["owned_cells"],
                    "launches": launches},
-        "wall_s": wall,
+        "wall_s": r["wall"],
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -207,38 +275,34 @@ def main():
                      "units_per_launch": per_launch_units,
                      "launch_ms": launch_ms},
         "compute": {"flops_per_cell_update": fpu,
-                    "achieved_tflops": fpu * value / 1e12,
+                    "achieved_tflops": fpu * r["value"] / 1e12,
                     "fp64_peak_tflops": FP64_PEAK_TFLOPS,
-                    "frac": fpu * value / 1e12 / FP64_PEAK_TFLOPS},
+                    "frac": fpu * r["value"] / 1e12 / FP64_PEAK_TFLOPS},
+        "forms": {f: {"value": results[f]["value"],
+                      "max_rel_err_vs_fp64": (max(results[f]["errs"])
+                                              if "errs" in results[f]
+                                              else None),
+                      "rel_err_per_record": results[f].get("errs"),
+                      "temporal_block":
+                          results[f]["info"]["temporal_block"],
+                      "tile_cells": results[f]["info"]["tile_cells"]}
+                  for f in forms},
     }
     traffic_file = os.path.join(ROOT, "profiles", "traffic_%s_%s.json"
-                                % (args.config, args.form))
+                                % (args.config, head))
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             t = json.load(f)
-        out["roofline"]["traffic"] = t.get("bytes_per_launch")
-
-    if not args.no_fp64 and args.form != "fp64":
-        log("building fp64 solver")
-        sol64 = build_solver(specs, "", **opts)
-        dev64, wall64 = timed_run(sol64, warmup, steps)
-        v64 = cu / (dev64 * 1e-3)
-        log("fp64: %.3f ms device, %.4g cu/s" % (dev64, v64))
-        rec64 = results_of(sol64)
-        del sol64
-        errs = []
-        for a, b in zip(rec_mixed, rec64):
-            scale = np.max(np.abs(b))
-            errs.append(float(np.max(np.abs(a - b)) / scale) if scale > 0
-                        else float(np.max(np.abs(a - b))))
-        out["fp64_value"] = v64
-        out["speedup_vs_fp64"] = value / v64
-        out["max_rel_err_vs_fp64"] = {"records": errs,
-                                      "max": max(errs) if errs else None}
-
+        if (t.get("temporal_block") == info["temporal_block"] and
+                t.get("tile_cells") == info["tile_cells"]):
+            out["roofline"]["traffic"] = t.get("bytes_per_launch")
+    if ref is not None:
+        out["fp64_value"] = ref["value"]
+        out["speedup_vs_fp64"] = r["value"] / ref["value"]
+        out["max_rel_err_vs_fp64"] = max(r["errs"]) if r["errs"] else 0.0
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(specs, args.form,
+        out["cpu_baseline"] = cpu_baseline(specs, head,
                                            seconds=args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
