@@ -47,6 +47,14 @@ def build(spec, clear_library=True):
         mb.material.clear_library()
     mats = {}
     for mid, m in spec["materials"].items():
+        if not clear_library:
+            # runs of one batch share materials of the same name (e.g. the
+            # vacuum); the first definition stays in the library
+            try:
+                mats[mid] = mb.material.get_from_library(mid)
+                continue
+            except Exception:
+                pass
         qm = _make_qm(mb, m.get("qm"))
         mat = mb.material(mid, qm, m.get("rel_permittivity", 1.0),
                           m.get("overlap_factor", 1.0),
