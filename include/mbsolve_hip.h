@@ -73,11 +73,14 @@ enum {
      * Results are bit-for-bit independent of tiling and temporal block. */
     MBH_FORM_FP64 = 0,
     /* FP32 state storage in HBM, FP64 arithmetic in registers.  Rodrigues
-     * runs in the rotated basis (z-form, needs zform_consts); Cayley and
-     * noop keep the reference update. */
+     * runs in the rotated basis (z-form, needs zform_consts); two-level
+     * Cayley runs in closed Bloch form (the Cayley factor applied as a
+     * rotation of the Bloch vector, constants derived from qm_consts);
+     * three-level Cayley and noop keep the reference update. */
     MBH_FORM_MIXED = 1,
-    /* FP64 state storage, z-form Rodrigues (same arithmetic as MIXED
-     * without the FP32 rounding of the stored state). */
+    /* FP64 state storage, same arithmetic as MIXED (z-form Rodrigues,
+     * Bloch-form two-level Cayley) without the FP32 rounding of the stored
+     * state. */
     MBH_FORM_ZFORM64 = 2
 };
 

@@ -92,7 +92,7 @@ parse_hip_solver_name(const std::string& name_in, density_algo& algo,
     }
     if (alg == "reg-cayley" && (num_lvl == 2 || num_lvl == 3)) {
         algo = density_algo::reg_cayley;
-        return form != MBH_FORM_ZFORM64;
+        return num_lvl == 2 || form != MBH_FORM_ZFORM64;
     }
     return false;
 }

@@ -8,6 +8,7 @@
  */
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -40,6 +41,90 @@ struct run_layout {
     uint32_t dstride_ref; /* reference density stride */
 };
 
+/* Components of the reference 2x2 density layout (re/im interleaved,
+ * row-major) kept by the Bloch form: rho00, rho11, Re rho01, Im rho01. */
+constexpr uint32_t BLOCH_FROM_REF[4] = { 0, 6, 2, 3 };
+
+/* Bloch-form constants (layout in algo_bloch2) from the reference Cayley
+ * blob of a two-level material (MBH_CAYLEY_STRIDE(2) doubles). */
+void bloch2_constants(const double* cq, double* o)
+{
+    typedef std::complex<double> cx;
+    std::fill(o, o + BLOCH2_STRIDE, 0.0);
+    if (cq[0] == 0.0) {
+        return;
+    }
+    const double carrier = cq[1], prop_im = cq[3], rhs_im = cq[4];
+    cx H0[4], mu[4];
+    for (int i = 0; i < 4; i++) {
+        H0[i] = cx(cq[5 + 2 * i], cq[6 + 2 * i]);
+        mu[i] = cx(cq[13 + 2 * i], cq[14 + 2 * i]);
+    }
+    const double* r1 = cq + 21;
+    const double* r2 = cq + 25;
+    const double* D1 = cq + 29;
+    const double* D2 = cq + 33;
+    /* Pauli components of a Hermitian 2x2: X = x0 I + x.sigma */
+    auto pauli = [](const cx* X, double* v) {
+        v[0] = 0.5 * (X[0].real() + X[3].real());
+        v[1] = X[1].real();
+        v[2] = -X[1].imag();
+        v[3] = 0.5 * (X[0].real() - X[3].real());
+    };
+    double h[4], m[4];
+    pauli(H0, h);
+    pauli(mu, m);
+    /* H = H0 - e mu; beta = -prop_im h_vec, alpha0 = -(prop_im / 2) h0 */
+    o[0] = 1.0;
+    for (int j = 0; j < 3; j++) {
+        o[1 + j] = -prop_im * h[1 + j];
+        o[4 + j] = prop_im * m[1 + j];
+    }
+    o[7] = -0.5 * prop_im * h[0];
+    o[8] = 0.5 * prop_im * m[0];
+    o[9] = D2[1];
+    for (int j = 0; j < 4; j++) {
+        o[10 + j] = D1[j];
+    }
+    /* p = carrier Re tr[(-i/hbar [H0, d] + D(d)) mu], evaluated on the
+     * basis states of (rho00, rho11, Re rho01, Im rho01) */
+    const cx I(0.0, 1.0);
+    for (int b = 0; b < 4; b++) {
+        cx d[4] = { 0.0, 0.0, 0.0, 0.0 };
+        if (b == 0) {
+            d[0] = 1.0;
+        } else if (b == 1) {
+            d[3] = 1.0;
+        } else if (b == 2) {
+            d[1] = 1.0;
+            d[2] = 1.0;
+        } else {
+            d[1] = I;
+            d[2] = -I;
+        }
+        cx rhs[4];
+        for (int i = 0; i < 2; i++) {
+            for (int k = 0; k < 2; k++) {
+                cx c = 0.0;
+                for (int j = 0; j < 2; j++) {
+                    c += H0[i * 2 + j] * d[j * 2 + k] -
+                        d[i * 2 + j] * H0[j * 2 + k];
+                }
+                cx dis = (i == k) ? r1[i * 2] * d[0] + r1[i * 2 + 1] * d[3]
+                                  : r2[i * 2 + k] * d[i * 2 + k];
+                rhs[i * 2 + k] = I * rhs_im * c + dis;
+            }
+        }
+        double tr = 0.0;
+        for (int i = 0; i < 2; i++) {
+            for (int k = 0; k < 2; k++) {
+                tr += (rhs[i * 2 + k] * mu[k * 2 + i]).real();
+            }
+        }
+        o[14 + b] = carrier * tr;
+    }
+}
+
 } /* namespace */
 
 struct mbh_ctx {
@@ -55,6 +140,7 @@ struct mbh_ctx {
     uint32_t dref = 0;      /* reference density components per cell */
     bool fp32 = false;
     bool zform = false;
+    bool bloch = false; /* two-level Cayley in closed Bloch form */
 
     uint64_t G = 0, dstride = 0;
     std::vector<run_layout> runs;
@@ -324,10 +410,14 @@ const kernel_entry* find_kernel(uint32_t algo, uint32_t levels, uint32_t form,
     return nullptr;
 }
 
-/* Kernel form implementing the requested numerical form. */
-uint32_t kernel_form(uint32_t algo, uint32_t form)
+/* Kernel form implementing the requested numerical form: Rodrigues and
+ * two-level Cayley have restructured kernels for MIXED and ZFORM64; the
+ * others fall back to the reference operation order (ZFORM64 -> FP64). */
+uint32_t kernel_form(uint32_t algo, uint32_t levels, uint32_t form)
 {
-    if (algo != MBH_ALGO_CVR_RODR && form == MBH_FORM_ZFORM64) {
+    const bool restructured = (algo == MBH_ALGO_CVR_RODR) ||
+        (algo == MBH_ALGO_REG_CAYLEY && levels == 2);
+    if (!restructured && form == MBH_FORM_ZFORM64) {
         return MBH_FORM_FP64;
     }
     return form;
@@ -426,7 +516,7 @@ KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
  */
 int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
 {
-    const uint32_t kform = kernel_form(ctx->algo, ctx->form);
+    const uint32_t kform = kernel_form(ctx->algo, ctx->levels, ctx->form);
     std::vector<uint32_t> cs;
     for (uint32_t C : { 1u, 2u, 4u }) {
         if (find_kernel(ctx->algo, ctx->levels, kform, C) &&
@@ -669,6 +759,11 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     c->fp32 = (opt.form == MBH_FORM_MIXED);
     c->zform = (c->algo == MBH_ALGO_CVR_RODR) &&
         (opt.form == MBH_FORM_MIXED || opt.form == MBH_FORM_ZFORM64);
+    c->bloch = (c->algo == MBH_ALGO_REG_CAYLEY) && c->levels == 2 &&
+        (opt.form == MBH_FORM_MIXED || opt.form == MBH_FORM_ZFORM64);
+    if (c->bloch) {
+        c->DC = 4;
+    }
     const size_t esz = c->fp32 ? sizeof(float) : sizeof(double);
 
     /* super-grid layout */
@@ -693,7 +788,9 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     std::vector<int32_t> mat(G, -1);
     std::vector<uint8_t> hz(G, 1);
     std::vector<double> fdtd((size_t)nmat_total * FDTD_STRIDE);
-    c->qstride = c->zform ? runs[0].zform_stride : runs[0].qm_stride;
+    c->qstride = c->bloch ? BLOCH2_STRIDE
+                          : (c->zform ? runs[0].zform_stride
+                                      : runs[0].qm_stride);
     std::vector<double> qm((size_t)nmat_total * c->qstride);
     uint32_t moff = 0;
     for (uint32_t r = 0; r < num_runs; r++) {
@@ -710,7 +807,10 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
             }
             fdtd[(size_t)(moff + m) * FDTD_STRIDE + 4] = p.dx_inv;
             const double* q = c->zform ? p.zform_consts : p.qm_consts;
-            if (c->qstride) {
+            if (c->bloch) {
+                bloch2_constants(q + (size_t)m * p.qm_stride,
+                                 &qm[(size_t)(moff + m) * c->qstride]);
+            } else if (c->qstride) {
                 std::memcpy(&qm[(size_t)(moff + m) * c->qstride],
                             q + (size_t)m * c->qstride,
                             c->qstride * sizeof(double));
@@ -724,7 +824,8 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
      * parity is the same before and after a replay */
     c->graph_launches =
         (opt.graph_launches > 1) ? ((opt.graph_launches + 1u) & ~1u) : 1u;
-    if (!find_kernel(c->algo, c->levels, kernel_form(c->algo, c->form), 1)) {
+    if (!find_kernel(c->algo, c->levels,
+                     kernel_form(c->algo, c->levels, c->form), 1)) {
         return fail(nullptr, MBH_ERR_UNSUPPORTED,
                     "no kernel for algo %u, %u levels, form %u", c->algo,
                     c->levels, c->form);
@@ -793,6 +894,17 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
                 }
             }
             h[L.base + L.cells] = p.h_init[L.cells];
+        }
+        if (c->bloch) {
+            /* keep rho00, rho11 and rho01 of the Hermitian state */
+            std::vector<double> sel((size_t)c->DC * c->dstride);
+            for (uint32_t j = 0; j < c->DC; j++) {
+                std::copy(d.begin() + (size_t)BLOCH_FROM_REF[j] * c->dstride,
+                          d.begin() +
+                              (size_t)(BLOCH_FROM_REF[j] + 1) * c->dstride,
+                          sel.begin() + (size_t)j * c->dstride);
+            }
+            d.swap(sel);
         }
         if (c->fp32) {
             std::vector<float> ef(e.begin(), e.end()), hf(h.begin(), h.end());
@@ -1033,7 +1145,24 @@ int mbh_fetch_state(mbh_ctx* ctx, uint32_t run, double* e, double* h,
     }
     if (d && ctx->dref) {
         std::vector<double> comp(L.cells);
-        if (ctx->zform) {
+        if (ctx->bloch) {
+            /* Hermitian 2x2 from rho00, rho11, rho01 */
+            std::fill(d, d + L.cells * ctx->dref, 0.0);
+            for (uint32_t j = 0; j < ctx->DC; j++) {
+                if ((rc = get(S[3], (uint64_t)j * ctx->dstride + L.base,
+                              L.cells, comp.data()))) {
+                    return rc;
+                }
+                for (uint64_t i = 0; i < L.cells; i++) {
+                    d[i * ctx->dref + BLOCH_FROM_REF[j]] = comp[i];
+                    if (j == 2) {
+                        d[i * ctx->dref + 4] = comp[i];
+                    } else if (j == 3) {
+                        d[i * ctx->dref + 5] = -comp[i];
+                    }
+                }
+            }
+        } else if (ctx->zform) {
             double* dtmp = nullptr;
             rc = dalloc(ctx, (void**)&dtmp,
                         (size_t)ctx->dref * ctx->dstride * sizeof(double));

@@ -333,6 +333,108 @@ struct algo_cayley {
     }
 };
 
+/* ======================================================================
+ * Two-level regularized Cayley in closed Bloch form.  Mathematically the
+ * update of algo_cayley<2> (algo_lindblad_reg_cayley.hpp:206-257) for a
+ * Hermitian state, rearranged: with H = h0 I + h.sigma and
+ * alpha = -(prop_im / 2) (h0, h), the Cayley factor is proportional to
+ *   W = s I - i beta.sigma,  beta = 2 alpha_vec,  s = 1 + alpha0^2 - |alpha|^2
+ * so U rho U^+ rotates the Bloch vector m~ = (Re rho01, -Im rho01,
+ * (rho00 - rho11) / 2) by
+ *   m~' = m~ + lambda (s beta x m~ + beta x (beta x m~)),
+ *   lambda = 2 / (s^2 + |beta|^2)
+ * and leaves the trace unchanged.  Dissipation scales the coherence by g and
+ * maps the populations with D1; p is a fixed linear form of the state.
+ * State (4 values): rho00, rho11, Re rho01, Im rho01.
+ * Blob (MBH_BLOCH2 layout, 18 doubles):
+ *   [0] has_qm  [1..3] beta at e = 0  [4..6] d beta / d e
+ *   [7] alpha0 at e = 0  [8] d alpha0 / d e  [9] g
+ *   [10..13] D1 (row-major, populations)  [14..17] p coefficients of
+ *   (rho00, rho11, Re rho01, Im rho01)
+ * ==================================================================== */
+struct algo_bloch2 {
+    static constexpr int DC = 4;
+    static constexpr int STATE = 4;
+    struct state {
+        double v[STATE];
+    };
+
+    __device__ static bool has_qm(cdp q) { return q[0] != 0.0; }
+
+    /* 2 / n2 for n2 near 1: hardware reciprocal and two Newton steps */
+    __device__ static double two_over(double n2)
+    {
+        double r = __builtin_amdgcn_rcp(n2);
+        double t = fma(-n2, r, 1.0);
+        r = fma(r, t, r);
+        t = fma(-n2, r, 1.0);
+        r = fma(r, t, r);
+        return 2.0 * r;
+    }
+
+    template<int C>
+    __device__ static void update_multi(cdp q, state* s, const double* e,
+                                        double* p)
+    {
+        const double bcx = q[1], bcy = q[2], bcz = q[3];
+        const double bex = q[4], bey = q[5], bez = q[6];
+        const double a0c = q[7], a0e = q[8], g = q[9];
+        const double D00 = q[10], D01 = q[11], D10 = q[12], D11 = q[13];
+        const double pa = q[14], pb = q[15], px = q[16], py = q[17];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            double* v = s[c].v;
+            /* first dissipation half step */
+            const double a1 = fma(D01, v[1], D00 * v[0]);
+            const double b1 = fma(D11, v[1], D10 * v[0]);
+            const double mx = g * v[2];
+            const double my = -(g * v[3]);
+            const double mz = 0.5 * (a1 - b1);
+            const double tr = a1 + b1;
+            /* Cayley rotation at field e */
+            const double bx = fma(e[c], bex, bcx);
+            const double by = fma(e[c], bey, bcy);
+            const double bz = fma(e[c], bez, bcz);
+            const double a0 = fma(e[c], a0e, a0c);
+            const double r2 = fma(bz, bz, fma(by, by, bx * bx));
+            const double sc = fma(-0.25, r2, fma(a0, a0, 1.0));
+            const double lam = two_over(fma(sc, sc, r2));
+            const double c1x = fma(by, mz, -(bz * my));
+            const double c1y = fma(bz, mx, -(bx * mz));
+            const double c1z = fma(bx, my, -(by * mx));
+            const double bm = fma(bz, mz, fma(by, my, bx * mx));
+            const double c2x = fma(bx, bm, -(mx * r2));
+            const double c2y = fma(by, bm, -(my * r2));
+            const double c2z = fma(bz, bm, -(mz * r2));
+            const double nx = fma(lam, fma(sc, c1x, c2x), mx);
+            const double ny = fma(lam, fma(sc, c1y, c2y), my);
+            const double nz = fma(lam, fma(sc, c1z, c2z), mz);
+            const double a2 = fma(0.5, tr, nz);
+            const double b2 = fma(0.5, tr, -nz);
+            /* second dissipation half step */
+            v[0] = fma(D01, b2, D00 * a2);
+            v[1] = fma(D11, b2, D10 * a2);
+            v[2] = g * nx;
+            v[3] = -(g * ny);
+            p[c] = fma(py, v[3], fma(px, v[2], fma(pb, v[1], pa * v[0])));
+        }
+    }
+
+    __device__ static void update(cdp q, state& s, double e, double& p)
+    {
+        update_multi<1>(q, &s, &e, &p);
+    }
+
+    __device__ static double inversion(cdp, const state& s)
+    {
+        return s.v[1] - s.v[0];
+    }
+    __device__ static double population(cdp, const state& s, int m)
+    {
+        return (m == 0) ? s.v[0] : ((m == 1) ? s.v[1] : 0.0);
+    }
+};
+
 /* ---- coherence-vector population readout ------------------------------
  * cv_representation::calc_population
  * (coherence_vector_representation.hpp:289-336). */

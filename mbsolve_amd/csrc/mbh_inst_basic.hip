@@ -1,6 +1,8 @@
 /*
  * Kernel instantiations: noop (pure Maxwell) and regularized Cayley.
- * FP64 storage implements MBH_FORM_FP64, FP32 storage MBH_FORM_MIXED.
+ * Reference operation order: FP64 storage implements MBH_FORM_FP64, FP32
+ * storage MBH_FORM_MIXED (noop, three levels).  Two-level Cayley in closed
+ * Bloch form: FP32 storage MBH_FORM_MIXED, FP64 storage MBH_FORM_ZFORM64.
  */
 #include "mbh_tile.hpp"
 
@@ -18,9 +20,12 @@ template __global__ void tile_kernel<algo_noop, 4, float>(KArgs);
 template __global__ void tile_kernel<algo_cayley<2>, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<2>, 2, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<2>, 4, double>(KArgs);
-template __global__ void tile_kernel<algo_cayley<2>, 1, float>(KArgs);
-template __global__ void tile_kernel<algo_cayley<2>, 2, float>(KArgs);
-template __global__ void tile_kernel<algo_cayley<2>, 4, float>(KArgs);
+template __global__ void tile_kernel<algo_bloch2, 1, float>(KArgs);
+template __global__ void tile_kernel<algo_bloch2, 2, float>(KArgs);
+template __global__ void tile_kernel<algo_bloch2, 4, float>(KArgs);
+template __global__ void tile_kernel<algo_bloch2, 1, double>(KArgs);
+template __global__ void tile_kernel<algo_bloch2, 2, double>(KArgs);
+template __global__ void tile_kernel<algo_bloch2, 4, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, float>(KArgs);
 
@@ -36,9 +41,12 @@ const kernel_entry k_table_basic[] = {
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 1, algo_cayley<2>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 2, algo_cayley<2>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 4, algo_cayley<2>, double),
-    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 1, algo_cayley<2>, float),
-    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 2, algo_cayley<2>, float),
-    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 4, algo_cayley<2>, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 1, algo_bloch2, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 2, algo_bloch2, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 4, algo_bloch2, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 1, algo_bloch2, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 2, algo_bloch2, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 4, algo_bloch2, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_FP64, 1, algo_cayley<3>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_MIXED, 1, algo_cayley<3>, float),
 };

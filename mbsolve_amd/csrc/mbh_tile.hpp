@@ -28,6 +28,7 @@
 namespace mbh {
 
 constexpr int FDTD_STRIDE = 5; /* fac_a fac_b fac_c gamma dx_inv */
+constexpr int BLOCH2_STRIDE = 18; /* two-level Cayley, Bloch form */
 
 struct dev_source {
     uint64_t gx;         /* super-grid cell */

@@ -80,6 +80,36 @@ def test_zform64_close_to_reference_form(gpu):
         assert rel_err(a, b) <= 1e-10, rel_err(a, b)
 
 
+@pytest.mark.parametrize("form,tol", [("zform", 1e-11), ("mixed", 1e-5)])
+def test_cayley_bloch_forms(gpu, form, tol):
+    """Two-level Cayley in closed Bloch form (FP64 / FP32 state) against
+    the oracle's reference-order update: SIT pulse with finite T1/T2 and
+    a soft source, every record kind."""
+    spec = setups.sit_2lvl(num_gridpoints=3000, endtime=1.0e-12, t1=5e-13,
+                           t2=2e-13, num_records=40)
+    spec["records"] += [
+        {"name": "d11", "type": 3, "row": 1, "col": 1, "interval": 0.0,
+         "position": 75e-6},
+        {"name": "d22", "type": 3, "row": 2, "col": 2, "interval": 0.0,
+         "position": 75e-6},
+        {"name": "p", "interval": 1e-12 / 39},
+        {"name": "h", "interval": 1e-12 / 39}]
+    sol, recs = _run_hip(spec, form)
+    ref = _oracle_records(sol)
+    for r, a, b in zip(sol.get_results(), recs, ref):
+        if r.get_name().startswith("d"):
+            err = float(np.max(np.abs(a - b)))
+        else:
+            err = rel_err(a, b)
+        assert err <= tol, (r.get_name(), err)
+    st = sol.fetch_state()
+    rs = oracle.OracleRun(sol.problem()).run().state()
+    scale = {"e": np.max(np.abs(rs["e"])), "h": np.max(np.abs(rs["h"])),
+             "d": 1.0}
+    for k in ("e", "h", "d"):
+        assert np.max(np.abs(st[k] - rs[k])) <= tol * scale[k], k
+
+
 def test_capi_direct(gpu):
     spec = setups.sit_2lvl(num_gridpoints=1000, endtime=0.3e-12,
                            num_records=8)
