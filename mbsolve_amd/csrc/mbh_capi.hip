@@ -64,6 +64,7 @@ void bloch2_constants(const double* cq, double* o)
     const double* r2 = cq + 25;
     const double* D1 = cq + 29;
     const double* D2 = cq + 33;
+    const double g = D2[1];
     /* Pauli components of a Hermitian 2x2: X = x0 I + x.sigma */
     auto pauli = [](const cx* X, double* v) {
         v[0] = 0.5 * (X[0].real() + X[3].real());
@@ -82,13 +83,25 @@ void bloch2_constants(const double* cq, double* o)
     }
     o[7] = -0.5 * prop_im * h[0];
     o[8] = 0.5 * prop_im * m[0];
-    o[9] = D2[1];
-    for (int j = 0; j < 4; j++) {
-        o[10 + j] = D1[j];
-    }
-    /* p = carrier Re tr[(-i/hbar [H0, d] + D(d)) mu], evaluated on the
-     * basis states of (rho00, rho11, Re rho01, Im rho01) */
+    o[9] = g * g;
+    /* (a, b) = Ti (tau, m_z), (tau, m_z) = T (a, b) */
+    const double T[4] = { 1.0, 1.0, 0.5, -0.5 };
+    const double Ti[4] = { 0.5, 1.0, 0.5, -1.0 };
+    auto mul2 = [](const double* A, const double* B, double* C) {
+        C[0] = A[0] * B[0] + A[1] * B[2];
+        C[1] = A[0] * B[1] + A[1] * B[3];
+        C[2] = A[2] * B[0] + A[3] * B[2];
+        C[3] = A[2] * B[1] + A[3] * B[3];
+    };
+    double E[4], ETi[4], R[4];
+    mul2(D1, D1, E);
+    mul2(E, Ti, ETi);
+    mul2(T, ETi, o + 10); /* F = T D1^2 Ti */
+    mul2(D1, Ti, R);      /* populations after the step */
+    /* p = carrier Re tr[(-i/hbar [H0, d] + D(d)) mu] as a linear form of
+     * (rho00, rho11, Re rho01, Im rho01), evaluated on basis states */
     const cx I(0.0, 1.0);
+    double pc[4];
     for (int b = 0; b < 4; b++) {
         cx d[4] = { 0.0, 0.0, 0.0, 0.0 };
         if (b == 0) {
@@ -121,8 +134,52 @@ void bloch2_constants(const double* cq, double* o)
                 tr += (rhs[i * 2 + k] * mu[k * 2 + i]).real();
             }
         }
-        o[14 + b] = carrier * tr;
+        pc[b] = carrier * tr;
     }
+    /* rho_n = D(sigma): (rho00, rho11) = R (tau, m_z), rho01 = g (m_x - i m_y) */
+    o[14] = pc[0] * R[0] + pc[1] * R[2];
+    o[15] = pc[2] * g;
+    o[16] = -pc[3] * g;
+    o[17] = pc[0] * R[1] + pc[1] * R[3];
+    for (int j = 0; j < 4; j++) {
+        o[18 + j] = R[j];
+        o[22 + j] = D1[j];
+    }
+    o[26] = g;
+}
+
+/* reference (rho00, rho11, Re rho01, Im rho01) -> carried Bloch state
+ * sigma with D(sigma) = rho */
+void bloch2_to_sigma(const double* o, const double* rho, double* sig)
+{
+    if (o[0] == 0.0) {
+        std::copy(rho, rho + 4, sig);
+        return;
+    }
+    const double* D1 = o + 22;
+    const double g = o[26];
+    const double det = D1[0] * D1[3] - D1[1] * D1[2];
+    const double a = (D1[3] * rho[0] - D1[1] * rho[1]) / det;
+    const double b = (D1[0] * rho[1] - D1[2] * rho[0]) / det;
+    sig[0] = a + b;
+    sig[1] = rho[2] / g;
+    sig[2] = -rho[3] / g;
+    sig[3] = 0.5 * (a - b);
+}
+
+/* carried Bloch state -> reference (rho00, rho11, Re rho01, Im rho01) */
+void bloch2_from_sigma(const double* o, const double* sig, double* rho)
+{
+    if (o[0] == 0.0) {
+        std::copy(sig, sig + 4, rho);
+        return;
+    }
+    const double* R = o + 18;
+    const double g = o[26];
+    rho[0] = R[0] * sig[0] + R[1] * sig[3];
+    rho[1] = R[2] * sig[0] + R[3] * sig[3];
+    rho[2] = g * sig[1];
+    rho[3] = -g * sig[2];
 }
 
 } /* namespace */
@@ -141,6 +198,8 @@ struct mbh_ctx {
     bool fp32 = false;
     bool zform = false;
     bool bloch = false; /* two-level Cayley in closed Bloch form */
+    std::vector<int32_t> mat_h; /* per super-grid cell (Bloch conversions) */
+    std::vector<double> qm_h;   /* device blob copy (Bloch conversions) */
 
     uint64_t G = 0, dstride = 0;
     std::vector<run_layout> runs;
@@ -896,15 +955,25 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
             h[L.base + L.cells] = p.h_init[L.cells];
         }
         if (c->bloch) {
-            /* keep rho00, rho11 and rho01 of the Hermitian state */
-            std::vector<double> sel((size_t)c->DC * c->dstride);
-            for (uint32_t j = 0; j < c->DC; j++) {
-                std::copy(d.begin() + (size_t)BLOCH_FROM_REF[j] * c->dstride,
-                          d.begin() +
-                              (size_t)(BLOCH_FROM_REF[j] + 1) * c->dstride,
-                          sel.begin() + (size_t)j * c->dstride);
+            /* rho00, rho11 and rho01 of the Hermitian state, converted to
+             * the carried Bloch state of the cell's material */
+            std::vector<double> sel((size_t)c->DC * c->dstride, 0.0);
+            for (uint64_t g = 0; g < G; g++) {
+                if (mat[g] < 0) {
+                    continue;
+                }
+                double rho[4], sig[4];
+                for (uint32_t j = 0; j < 4; j++) {
+                    rho[j] = d[(size_t)BLOCH_FROM_REF[j] * c->dstride + g];
+                }
+                bloch2_to_sigma(&qm[(size_t)mat[g] * c->qstride], rho, sig);
+                for (uint32_t j = 0; j < 4; j++) {
+                    sel[(size_t)j * c->dstride + g] = sig[j];
+                }
             }
             d.swap(sel);
+            c->mat_h = mat;
+            c->qm_h = qm;
         }
         if (c->fp32) {
             std::vector<float> ef(e.begin(), e.end()), hf(h.begin(), h.end());
@@ -1146,21 +1215,30 @@ int mbh_fetch_state(mbh_ctx* ctx, uint32_t run, double* e, double* h,
     if (d && ctx->dref) {
         std::vector<double> comp(L.cells);
         if (ctx->bloch) {
-            /* Hermitian 2x2 from rho00, rho11, rho01 */
-            std::fill(d, d + L.cells * ctx->dref, 0.0);
-            for (uint32_t j = 0; j < ctx->DC; j++) {
+            /* Hermitian 2x2 from the carried Bloch state */
+            std::vector<double> sig((size_t)4 * L.cells);
+            for (uint32_t j = 0; j < 4; j++) {
                 if ((rc = get(S[3], (uint64_t)j * ctx->dstride + L.base,
-                              L.cells, comp.data()))) {
+                              L.cells, sig.data() + (size_t)j * L.cells))) {
                     return rc;
                 }
-                for (uint64_t i = 0; i < L.cells; i++) {
-                    d[i * ctx->dref + BLOCH_FROM_REF[j]] = comp[i];
-                    if (j == 2) {
-                        d[i * ctx->dref + 4] = comp[i];
-                    } else if (j == 3) {
-                        d[i * ctx->dref + 5] = -comp[i];
-                    }
+            }
+            std::fill(d, d + L.cells * ctx->dref, 0.0);
+            for (uint64_t i = 0; i < L.cells; i++) {
+                const int32_t m = ctx->mat_h[L.base + i];
+                double sv[4], rho[4];
+                for (uint32_t j = 0; j < 4; j++) {
+                    sv[j] = sig[(size_t)j * L.cells + i];
                 }
+                bloch2_from_sigma(&ctx->qm_h[(size_t)m * ctx->qstride], sv,
+                                  rho);
+                double* di = d + i * ctx->dref;
+                di[0] = rho[0];
+                di[6] = rho[1];
+                di[2] = rho[2];
+                di[3] = rho[3];
+                di[4] = rho[2];
+                di[5] = -rho[3];
             }
         } else if (ctx->zform) {
             double* dtmp = nullptr;

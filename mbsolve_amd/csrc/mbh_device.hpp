@@ -165,6 +165,7 @@ __device__ __forceinline__ void cinverse(const cplx* A, cplx* R)
  * lindblad_noop (algo_lindblad_noop.hpp:35-86): no density, p untouched.
  * ==================================================================== */
 struct algo_noop {
+    static constexpr bool FAST_FDTD = false;
     static constexpr int DC = 0; /* stored density components */
     static constexpr int STATE = 1;
     struct state {
@@ -192,6 +193,7 @@ struct algo_noop {
  * ==================================================================== */
 template<int N>
 struct algo_cayley {
+    static constexpr bool FAST_FDTD = false;
     static constexpr int DC = 2 * N * N;
     static constexpr int STATE = 2 * N * N;
     struct state {
@@ -336,25 +338,33 @@ struct algo_cayley {
 /* ======================================================================
  * Two-level regularized Cayley in closed Bloch form.  Mathematically the
  * update of algo_cayley<2> (algo_lindblad_reg_cayley.hpp:206-257) for a
- * Hermitian state, rearranged: with H = h0 I + h.sigma and
- * alpha = -(prop_im / 2) (h0, h), the Cayley factor is proportional to
+ * Hermitian state, rearranged.  With H = h0 I + h.sigma and
+ * alpha = -(prop_im / 2) (h0, h_vec), the Cayley factor is proportional to
  *   W = s I - i beta.sigma,  beta = 2 alpha_vec,  s = 1 + alpha0^2 - |alpha|^2
- * so U rho U^+ rotates the Bloch vector m~ = (Re rho01, -Im rho01,
- * (rho00 - rho11) / 2) by
- *   m~' = m~ + lambda (s beta x m~ + beta x (beta x m~)),
+ * so U rho U^+ rotates m = (Re rho01, -Im rho01, (rho00 - rho11) / 2) by
+ *   m' = m + lambda (s beta x m + beta x (beta x m)),
  *   lambda = 2 / (s^2 + |beta|^2)
- * and leaves the trace unchanged.  Dissipation scales the coherence by g and
- * maps the populations with D1; p is a fixed linear form of the state.
- * State (4 values): rho00, rho11, Re rho01, Im rho01.
- * Blob (MBH_BLOCH2 layout, 18 doubles):
+ * and keeps the trace tau.  The dissipation D scales the coherence by g and
+ * maps the populations with D1.
+ *
+ * Carried state: sigma = the state between the unitary step and the second
+ * dissipation half step, as (tau, m_x, m_y, m_z).  The reference state
+ * after step n is rho_n = D(sigma_n), and the next step needs
+ * D(rho_n) = D^2(sigma_n): one merged dissipation per step.  p and the
+ * populations are fixed linear forms of sigma.
+ *
+ * Blob (BLOCH2_STRIDE doubles):
  *   [0] has_qm  [1..3] beta at e = 0  [4..6] d beta / d e
- *   [7] alpha0 at e = 0  [8] d alpha0 / d e  [9] g
- *   [10..13] D1 (row-major, populations)  [14..17] p coefficients of
- *   (rho00, rho11, Re rho01, Im rho01)
+ *   [7] alpha0 at e = 0  [8] d alpha0 / d e  [9] g^2
+ *   [10..13] F: D1^2 on (tau, m_z), row-major
+ *   [14..17] p = P . (tau, m_x, m_y, m_z)
+ *   [18..21] populations: rho00 = R00 tau + R01 m_z, rho11 = R10 tau + R11 m_z
+ *   [22..25] D1 (row-major)  [26] g  [27] 0
  * ==================================================================== */
 struct algo_bloch2 {
     static constexpr int DC = 4;
     static constexpr int STATE = 4;
+    static constexpr bool FAST_FDTD = true;
     struct state {
         double v[STATE];
     };
@@ -378,19 +388,18 @@ struct algo_bloch2 {
     {
         const double bcx = q[1], bcy = q[2], bcz = q[3];
         const double bex = q[4], bey = q[5], bez = q[6];
-        const double a0c = q[7], a0e = q[8], g = q[9];
-        const double D00 = q[10], D01 = q[11], D10 = q[12], D11 = q[13];
-        const double pa = q[14], pb = q[15], px = q[16], py = q[17];
+        const double a0c = q[7], a0e = q[8], g2 = q[9];
+        const double F00 = q[10], F01 = q[11], F10 = q[12], F11 = q[13];
+        const double P0 = q[14], P1 = q[15], P2 = q[16], P3 = q[17];
 #pragma unroll
         for (int c = 0; c < C; c++) {
             double* v = s[c].v;
-            /* first dissipation half step */
-            const double a1 = fma(D01, v[1], D00 * v[0]);
-            const double b1 = fma(D11, v[1], D10 * v[0]);
-            const double mx = g * v[2];
-            const double my = -(g * v[3]);
-            const double mz = 0.5 * (a1 - b1);
-            const double tr = a1 + b1;
+            /* second half step of the previous step and first half step
+             * of this one */
+            const double tr = fma(F01, v[3], F00 * v[0]);
+            const double mz = fma(F11, v[3], F10 * v[0]);
+            const double mx = g2 * v[1];
+            const double my = g2 * v[2];
             /* Cayley rotation at field e */
             const double bx = fma(e[c], bex, bcx);
             const double by = fma(e[c], bey, bcy);
@@ -406,17 +415,11 @@ struct algo_bloch2 {
             const double c2x = fma(bx, bm, -(mx * r2));
             const double c2y = fma(by, bm, -(my * r2));
             const double c2z = fma(bz, bm, -(mz * r2));
-            const double nx = fma(lam, fma(sc, c1x, c2x), mx);
-            const double ny = fma(lam, fma(sc, c1y, c2y), my);
-            const double nz = fma(lam, fma(sc, c1z, c2z), mz);
-            const double a2 = fma(0.5, tr, nz);
-            const double b2 = fma(0.5, tr, -nz);
-            /* second dissipation half step */
-            v[0] = fma(D01, b2, D00 * a2);
-            v[1] = fma(D11, b2, D10 * a2);
-            v[2] = g * nx;
-            v[3] = -(g * ny);
-            p[c] = fma(py, v[3], fma(px, v[2], fma(pb, v[1], pa * v[0])));
+            v[0] = tr;
+            v[1] = fma(lam, fma(sc, c1x, c2x), mx);
+            v[2] = fma(lam, fma(sc, c1y, c2y), my);
+            v[3] = fma(lam, fma(sc, c1z, c2z), mz);
+            p[c] = fma(P3, v[3], fma(P2, v[2], fma(P1, v[1], P0 * tr)));
         }
     }
 
@@ -425,13 +428,19 @@ struct algo_bloch2 {
         update_multi<1>(q, &s, &e, &p);
     }
 
-    __device__ static double inversion(cdp, const state& s)
+    __device__ static double population(cdp q, const state& s, int m)
     {
-        return s.v[1] - s.v[0];
+        if (m == 0) {
+            return fma(q[19], s.v[3], q[18] * s.v[0]);
+        }
+        if (m == 1) {
+            return fma(q[21], s.v[3], q[20] * s.v[0]);
+        }
+        return 0.0;
     }
-    __device__ static double population(cdp, const state& s, int m)
+    __device__ static double inversion(cdp q, const state& s)
     {
-        return (m == 0) ? s.v[0] : ((m == 1) ? s.v[1] : 0.0);
+        return population(q, s, 1) - population(q, s, 0);
     }
 };
 
@@ -481,6 +490,7 @@ __device__ __forceinline__ double cv_population(const double* d, int m)
  * ==================================================================== */
 template<int N>
 struct algo_rodr {
+    static constexpr bool FAST_FDTD = false;
     static constexpr int K = N * N - 1;
     static constexpr int B = K / 2;
     static constexpr int DC = K;
@@ -685,6 +695,7 @@ __device__ __forceinline__ void zsincos(double x, double& s, double& c)
  * ==================================================================== */
 template<int N>
 struct algo_zform {
+    static constexpr bool FAST_FDTD = true;
     static constexpr int K = N * N - 1;
     static constexpr int B = K / 2;
     static constexpr int DC = K;

@@ -28,7 +28,7 @@
 namespace mbh {
 
 constexpr int FDTD_STRIDE = 5; /* fac_a fac_b fac_c gamma dx_inv */
-constexpr int BLOCH2_STRIDE = 18; /* two-level Cayley, Bloch form */
+constexpr int BLOCH2_STRIDE = 28; /* two-level Cayley, Bloch form */
 
 struct dev_source {
     uint64_t gx;         /* super-grid cell */
@@ -239,6 +239,16 @@ __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
         } else {
             fa[c] = fb[c] = fc[c] = fg[c] = fx[c] = 0.0;
         }
+        if constexpr (A::FAST_FDTD) {
+            /* folded constants: e' = fa e + fg' p + fx' (h_r - h) with
+             * fg' = -fb fg, fx' = fb fx (zero outside the grid, so those
+             * cells stay zero); h' = hm h + fc' (e_c - e_l) with hm = 0,
+             * fc' = 0 where H is held at zero */
+            fg[c] = -(fb[c] * fg[c]);
+            fx[c] = fb[c] * fx[c];
+            fb[c] = hz[c] ? 0.0 : 1.0;
+            fc[c] = hz[c] ? 0.0 : fc[c];
+        }
     }
 
     /* material: one uniform material for the whole tile (the common case)
@@ -287,9 +297,14 @@ __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
 #pragma unroll
         for (int c = 0; c < C; c++) {
             const double hn = (c + 1 < C) ? h[(c + 1) % C] : h_right;
-            en[c] = (m[c] >= 0)
-                ? fa[c] * e[c] + fb[c] * (-fg[c] * p[c] + (hn - h[c]) * fx[c])
-                : 0.0;
+            if constexpr (A::FAST_FDTD) {
+                en[c] = fma(fx[c], hn - h[c], fma(fg[c], p[c], fa[c] * e[c]));
+            } else {
+                en[c] = (m[c] >= 0)
+                    ? fa[c] * e[c] +
+                        fb[c] * (-fg[c] * p[c] + (hn - h[c]) * fx[c])
+                    : 0.0;
+            }
         }
 
         /* sources, in list order (solver_cpu_fdtd.tpp:247-255) */
@@ -315,7 +330,11 @@ __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
 #pragma unroll
         for (int c = 0; c < C; c++) {
             const double ep = (c > 0) ? en[(c + C - 1) % C] : e_left;
-            h[c] = hz[c] ? 0.0 : h[c] + fc[c] * (en[c] - ep);
+            if constexpr (A::FAST_FDTD) {
+                h[c] = fma(fc[c], en[c] - ep, fb[c] * h[c]);
+            } else {
+                h[c] = hz[c] ? 0.0 : h[c] + fc[c] * (en[c] - ep);
+            }
         }
 
         /* density update (solver_cpu_fdtd.tpp:278-285) */
