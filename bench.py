@@ -1,22 +1,29 @@
 """Benchmark of the MI355X Maxwell-Bloch hot path.
 
-Workload (default): the three-level V system on the Rodrigues propagator,
-65536 cells over 150 um, run to 1.5 ps (392939 FDTD steps; mbsolve_amd.setups
-.v3_rodr).  A "step" is one FDTD time step of the whole grid: E update,
-sources, H update, density update of every cell, record copy.
+Workload (default, "v3"): the three-level V system on the Rodrigues
+propagator, 65536 cells over 150 um, run to 1.5 ps (392939 FDTD steps;
+mbsolve_amd.setups.v3_rodr).  A "step" is one FDTD time step of the whole
+grid: E update, sources, H update, density update of every cell, record
+copy.  Other workloads: "sit2" (two-level self-induced transparency,
+regularized Cayley, 8192 cells, 1.5 ps), "qcl6" (six-level gain medium,
+Rodrigues, 2^20 cells, the first 20000 of 4e6 steps), "sweep2" (512
+two-level runs of 2^12..2^17 cells, 49114 shared steps), "sweep6" (64
+six-level runs of 2^14..2^18 cells, the first 20000 of 4e6 steps).
 
-Candidate fast forms (default: "mixed" = FP32 state storage with FP64
-arithmetic and rotated-basis Rodrigues, "zform" = the same with FP64 storage)
-and the plain FP64 form (reference operation order) are timed on the same
-steps; the records of each fast form are compared with the FP64 form
-(relative error max|x - x_fp64| / max|x_fp64|, per record) and the headline
-value is the fastest form within the 1e-5 bound.  Device time comes from HIP events
-on the solver's stream; the wall clock brackets the same region after a
-device synchronisation.  A bounded sample of the CPU FP64 oracle (the
-restatement of the reference's cpu-fdtd solver) is timed on the host cores.
+Candidate fast forms are timed on the same steps and their records compared
+with an FP64 reference run (relative error max|x - x_ref| / max|x_ref|, per
+record); the headline value is the fastest form within the 1e-5 bound.  The
+reference is the plain FP64 form (reference operation order) or, for the
+six-level workloads, the FP64 z-form (same arithmetic as the plain form up
+to rounding, 1e-13 from the oracle); the plain FP64 form is then timed on
+--fp64-steps steps for its speed.  Device time comes from HIP events on the
+solver's stream; the wall clock brackets the same region after a device
+synchronisation.  A bounded sample of the CPU FP64 oracle (the restatement
+of the reference's cpu-fdtd solver) is timed on the host cores.
 
-    python bench.py [--steps K] [--warmup W] [--config v3|sit2|qcl6|sweep2]
-                    [--no-cpu] [--no-fp64]
+    python bench.py [--steps K] [--warmup W]
+                    [--config v3|sit2|qcl6|sweep2|sweep6]
+                    [--forms a,b] [--no-cpu] [--no-fp64]
 """
 import argparse
 import json
@@ -37,8 +44,22 @@ FP64_PEAK_TFLOPS = 78.6
 ALGO_BYTES = {("v3", "f32"): 80, ("v3", "f64"): 160,
               ("sit2", "f32"): 48, ("sit2", "f64"): 96,
               ("qcl6", "f32"): 296, ("qcl6", "f64"): 592,
-              ("sweep2", "f32"): 48, ("sweep2", "f64"): 96}
-ALGO_FLOPS = {"v3": 168, "sit2": 120, "qcl6": 2622, "sweep2": 120}
+              ("sweep2", "f32"): 48, ("sweep2", "f64"): 96,
+              ("sweep6", "f32"): 296, ("sweep6", "f64"): 592}
+ALGO_FLOPS = {"v3": 168, "sit2": 120, "qcl6": 2622, "sweep2": 120,
+              "sweep6": 2622}
+# per workload: default candidate forms, reference form for the errors,
+# default steps of the plain FP64 timing (0: same as the candidates)
+DEFAULTS = {"v3": ("mixed,zform", "", 0),
+            "sit2": ("mixed,zform", "", 0),
+            "sweep2": ("mixed,zform", "", 0),
+            "qcl6": ("mfma,mfma64,zform", "zform", 400),
+            "sweep6": ("mfma,mfma64,zform", "zform", 400)}
+MFMA_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA
+# v_mfma_f32_16x16x32_f16 per 16-cell block and step: 3 split products x
+# M-tiles x K chunks (six levels: 3 x 3 x 2)
+MFMA_PER_BLOCK = {6: 18, 5: 6, 4: 3}
+MFMA_FLOPS = 2 * 16 * 16 * 32
 
 
 def log(*a):
@@ -48,17 +69,21 @@ def log(*a):
 def make_specs(config):
     from mbsolve_amd import setups
     if config == "v3":
-        spec = setups.v3_rodr()
-        return [spec], "configs[1]: 3-level V system, Rodrigues, 65536 " \
-            "cells, 150 um, 1.5 ps"
+        return [setups.v3_rodr()], "3-level V system, Rodrigues, 65536 " \
+            "cells, 150 um, 1.5 ps (392939 steps)"
     if config == "sit2":
-        return [setups.sit_2lvl()], "configs[0]: 2-level SIT, " \
-            "reg-Cayley, 8192 cells, 1.5 ps"
+        return [setups.sit_2lvl()], "2-level SIT, regularized Cayley, " \
+            "8192 cells, 1.5 ps (49114 steps)"
     if config == "qcl6":
-        return [setups.qcl6(num_timesteps=20000)], "configs[2] shape: " \
-            "6-level, 2^20 cells (20000 of 4e6 steps)"
+        return [setups.qcl6(num_timesteps=20000)], "6-level gain medium, " \
+            "Rodrigues, 2^20 cells, first 20000 of 4e6 steps"
     if config == "sweep2":
-        return setups.sit_2lvl_sweep(), "configs[3]: 512-run 2-level sweep"
+        return setups.sit_2lvl_sweep(), "512 two-level SIT runs, " \
+            "2^12..2^17 cells, regularized Cayley, 49114 steps"
+    if config == "sweep6":
+        return setups.qcl6_sweep(num_timesteps=20000), "64 six-level " \
+            "gain-medium runs, 2^14..2^18 cells, Rodrigues, first 20000 " \
+            "of 4e6 steps"
     raise ValueError(config)
 
 
@@ -86,14 +111,29 @@ def results_of(sol):
     return [x.get_data_real() for x in sol.get_results()]
 
 
-def timed_run(sol, warmup, steps):
-    """Warm-up steps [0, W), then the timed steps [W, W + K)."""
+def timed_run(sol, warmup, steps, chunk_s=20.0):
+    """Warm-up steps [0, W), then the timed steps [W, W + K) in chunks of
+    about chunk_s seconds (one progress line each); returns the summed
+    device time of the chunks and the wall time of the timed region."""
     if warmup:
         sol.run_steps(0, warmup)
         sol.fetch_results()  # waits for the device
+        w_ms = sol.elapsed_ms()
+        per_step = w_ms / warmup
+    else:
+        per_step = 0.0
+    chunk = steps if per_step <= 0 else \
+        max(1, min(steps, int(chunk_s * 1e3 / per_step)))
+    dev_ms = 0.0
     t0 = time.perf_counter()
-    sol.run_steps(warmup, warmup + steps)
-    dev_ms = sol.elapsed_ms()  # synchronises on the stop event
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        sol.run_steps(warmup + done, warmup + done + n)
+        dev_ms += sol.elapsed_ms()  # synchronises on the stop event
+        done += n
+        if done < steps:
+            log("    %d of %d steps, %.1f ms device" % (done, steps, dev_ms))
     wall = time.perf_counter() - t0
     sol.fetch_results()
     return dev_ms, wall
@@ -184,10 +224,16 @@ def main():
     ap.add_argument("--steps", type=int, default=0,
                     help="timed steps (default: the rest of the run)")
     ap.add_argument("--warmup", type=int, default=2048)
-    ap.add_argument("--config", default="v3")
-    ap.add_argument("--forms", default="mixed,zform",
+    ap.add_argument("--config", default="v3", choices=sorted(DEFAULTS))
+    ap.add_argument("--forms", default=None,
                     help="candidate fast forms; the headline is the fastest "
-                         "one within the error bound of the FP64 form")
+                         "one within the error bound of the FP64 reference")
+    ap.add_argument("--ref", default=None,
+                    help="reference form for the errors: '' (plain FP64) "
+                         "or 'zform'")
+    ap.add_argument("--fp64-steps", type=int, default=None,
+                    help="steps of the plain FP64 timing when it is not "
+                         "the reference (default per workload)")
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--temporal-block", type=int, default=0)
     ap.add_argument("--tile-cells", type=int, default=0)
@@ -215,12 +261,17 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
 
+    d_forms, d_ref, d_fp64_steps = DEFAULTS[args.config]
+    forms = [f for f in (args.forms if args.forms is not None
+                         else d_forms).split(",") if f]
+    ref_form = args.ref if args.ref is not None else d_ref
+    fp64_steps = (args.fp64_steps if args.fp64_steps is not None
+                  else d_fp64_steps)
     specs, workload = make_specs(args.config)
     opts = dict(temporal_block=args.temporal_block,
                 tile_cells=args.tile_cells,
                 graph_launches=args.graph_launches)
 
-    forms = [f for f in args.forms.split(",") if f]
     results = {}
     for form in forms:
         log("measuring form", form, "on", workload)
@@ -230,14 +281,28 @@ def main():
             % (form, r["dev_ms"], r["info"]["launches"], r["value"],
                r["info"]))
     ref = None
+    fp64 = None
     if not args.no_fp64:
-        log("measuring the FP64 reference-order form")
-        ref = measure(specs, "", args.warmup, args.steps, opts)
-        log("  fp64: %.3f ms device, %.4g cu/s" % (ref["dev_ms"],
-                                                   ref["value"]))
+        if ref_form in results:
+            ref = results[ref_form]
+        else:
+            log("measuring the reference form '%s'" % ref_form)
+            ref = measure(specs, ref_form, args.warmup, args.steps, opts)
+            log("  ref: %.3f ms device, %.4g cu/s" % (ref["dev_ms"],
+                                                      ref["value"]))
         for form, r in results.items():
             r["errs"] = rel_errs(r["records"], ref["records"])
-            log("  %s max rel err vs fp64: %s" % (form, r["errs"]))
+            log("  %s max rel err vs the reference: %s"
+                % (form, ["%.3g" % x for x in r["errs"]]))
+        if ref_form == "":
+            fp64 = ref
+        else:
+            log("timing the plain FP64 form on %s steps"
+                % (fp64_steps or "the same"))
+            fp64 = measure(specs, "", min(args.warmup, 64),
+                           fp64_steps or args.steps, opts)
+            log("  fp64: %.3f ms device, %.4g cu/s" % (fp64["dev_ms"],
+                                                       fp64["value"]))
 
     # headline: fastest form within the error bound (if measured)
     ok = [f for f in forms
@@ -246,7 +311,7 @@ def main():
     head = max(pool, key=lambda f: results[f]["value"])
     r = results[head]
     info = r["info"]
-    storage = "f32" if head == "mixed" else "f64"
+    storage = "f32" if head in ("mixed", "mfma") else "f64"
     launches = int(info["launches"])
     per_launch_units = r["cells"] * info["temporal_block"]
     launch_ms = r["dev_ms"] / max(launches, 1)
@@ -279,15 +344,27 @@ def main():
                     "fp64_peak_tflops": FP64_PEAK_TFLOPS,
                     "frac": fpu * r["value"] / 1e12 / FP64_PEAK_TFLOPS},
         "forms": {f: {"value": results[f]["value"],
-                      "max_rel_err_vs_fp64": (max(results[f]["errs"])
-                                              if "errs" in results[f]
-                                              else None),
+                      "max_rel_err_vs_ref": (max(results[f]["errs"])
+                                             if "errs" in results[f]
+                                             else None),
                       "rel_err_per_record": results[f].get("errs"),
                       "temporal_block":
                           results[f]["info"]["temporal_block"],
                       "tile_cells": results[f]["info"]["tile_cells"]}
                   for f in forms},
+        "reference_form": ("fp64" if ref_form == "" else ref_form)
+        if ref is not None else None,
     }
+    if head.startswith("mfma"):
+        # matrix-core work of the uniform tiles (the few tiles that straddle
+        # runs or the grid ends take the VALU kernel)
+        lev = 6 if args.config in ("qcl6", "sweep6") else 0
+        blocks = 4 * info["num_tiles"] * launches * info["temporal_block"]
+        hw = blocks * MFMA_PER_BLOCK.get(lev, 0) * MFMA_FLOPS
+        t = hw / (r["dev_ms"] * 1e-3) / 1e12
+        out["mfma"] = {"hardware_tflops": t, "peak_tflops": MFMA_PEAK_TFLOPS,
+                       "frac": t / MFMA_PEAK_TFLOPS,
+                       "instr": "v_mfma_f32_16x16x32_f16, 3 split products"}
     traffic_file = os.path.join(ROOT, "profiles", "traffic_%s_%s.json"
                                 % (args.config, head))
     if os.path.exists(traffic_file):
@@ -296,10 +373,12 @@ def main():
         if (t.get("temporal_block") == info["temporal_block"] and
                 t.get("tile_cells") == info["tile_cells"]):
             out["roofline"]["traffic"] = t.get("bytes_per_launch")
+    if fp64 is not None:
+        out["fp64_value"] = fp64["value"]
+        out["fp64_steps"] = fp64["steps"]
+        out["speedup_vs_fp64"] = r["value"] / fp64["value"]
     if ref is not None:
-        out["fp64_value"] = ref["value"]
-        out["speedup_vs_fp64"] = r["value"] / ref["value"]
-        out["max_rel_err_vs_fp64"] = max(r["errs"]) if r["errs"] else 0.0
+        out["max_rel_err_vs_ref"] = max(r["errs"]) if r["errs"] else 0.0
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(specs, head,

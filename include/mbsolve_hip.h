@@ -81,7 +81,13 @@ enum {
     /* FP64 state storage, same arithmetic as MIXED (z-form Rodrigues,
      * Bloch-form two-level Cayley) without the FP32 rounding of the stored
      * state. */
-    MBH_FORM_ZFORM64 = 2
+    MBH_FORM_ZFORM64 = 2,
+    /* Rodrigues z-form, N = 4..6, FP32 state storage: rotation, affine part
+     * and p in FP64, the generator product (W - I) r on the matrix cores as
+     * FP16 hi + lo split products with FP32 accumulation. */
+    MBH_FORM_MFMA = 3,
+    /* as MBH_FORM_MFMA with FP64 state storage */
+    MBH_FORM_MFMA64 = 4
 };
 
 /* ---- record / source types (same values as mbsolve::record::type and

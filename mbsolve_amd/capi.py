@@ -18,8 +18,10 @@ MBH_ABI_VERSION = 1
 MBH_OK = 0
 MBH_ALGO_NOOP, MBH_ALGO_REG_CAYLEY, MBH_ALGO_CVR_RODR = 0, 1, 2
 MBH_FORM_FP64, MBH_FORM_MIXED, MBH_FORM_ZFORM64 = 0, 1, 2
+MBH_FORM_MFMA, MBH_FORM_MFMA64 = 3, 4
 FORMS = {"fp64": MBH_FORM_FP64, "mixed": MBH_FORM_MIXED,
-         "zform64": MBH_FORM_ZFORM64}
+         "zform64": MBH_FORM_ZFORM64, "mfma": MBH_FORM_MFMA,
+         "mfma64": MBH_FORM_MFMA64}
 
 EXPORTED = ["mbh_version", "mbh_device_count", "mbh_default_options",
             "mbh_create", "mbh_run", "mbh_synchronize", "mbh_elapsed_ms",

@@ -522,7 +522,7 @@ PYBIND11_MODULE(_mbsolve, m)
             }
             fdtd_problem p;
             const bool zf = (a == density_algo::cvr_rodr) &&
-                (form == MBH_FORM_MIXED || form == MBH_FORM_ZFORM64);
+                form != MBH_FORM_FP64;
             build_fdtd_problem(dev, scen, a, l, zf, p);
             return problem_dict(p);
         },

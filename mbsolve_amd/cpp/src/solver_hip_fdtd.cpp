@@ -71,11 +71,16 @@ parse_hip_solver_name(const std::string& name_in, density_algo& algo,
         form = MBH_FORM_MIXED;
     } else if (strip("-zform")) {
         form = MBH_FORM_ZFORM64;
+    } else if (strip("-mfma64")) {
+        form = MBH_FORM_MFMA64;
+    } else if (strip("-mfma")) {
+        form = MBH_FORM_MFMA;
     }
+    const bool mfma = (form == MBH_FORM_MFMA || form == MBH_FORM_MFMA64);
     if (rest == "noop") {
         algo = density_algo::noop;
         num_lvl = 0;
-        return form != MBH_FORM_ZFORM64;
+        return form == MBH_FORM_FP64 || form == MBH_FORM_MIXED;
     }
     const size_t p = rest.find("lvl-");
     if (p == std::string::npos || p == 0) {
@@ -88,11 +93,11 @@ parse_hip_solver_name(const std::string& name_in, density_algo& algo,
     num_lvl = lv[0] - '0';
     if (alg == "rodr") {
         algo = density_algo::cvr_rodr;
-        return true;
+        return !mfma || num_lvl >= 4;
     }
     if (alg == "reg-cayley" && (num_lvl == 2 || num_lvl == 3)) {
         algo = density_algo::reg_cayley;
-        return num_lvl == 2 || form != MBH_FORM_ZFORM64;
+        return !mfma && (num_lvl == 2 || form != MBH_FORM_ZFORM64);
     }
     return false;
 }
@@ -125,8 +130,7 @@ solver_hip_fdtd::solver_hip_fdtd(std::shared_ptr<const device> dev,
     if (!parse_hip_solver_name(name, algo, lvl, m_form)) {
         throw std::invalid_argument("Unknown solver " + name);
     }
-    const bool zf = (algo == density_algo::cvr_rodr) &&
-        (m_form == MBH_FORM_MIXED || m_form == MBH_FORM_ZFORM64);
+    const bool zf = (algo == density_algo::cvr_rodr) && m_form != MBH_FORM_FP64;
     build_fdtd_problem(dev, scen, algo, lvl, zf, m_problem);
     for (const auto& e : m_problem.copy_list) {
         m_results.push_back(e.res);
@@ -196,8 +200,7 @@ solver_hip_batch::solver_hip_batch(
     if (runs.empty()) {
         throw std::invalid_argument("Empty batch.");
     }
-    const bool zf = (algo == density_algo::cvr_rodr) &&
-        (form == MBH_FORM_MIXED || form == MBH_FORM_ZFORM64);
+    const bool zf = (algo == density_algo::cvr_rodr) && form != MBH_FORM_FP64;
     m_problems.resize(runs.size());
     for (size_t r = 0; r < runs.size(); r++) {
         build_fdtd_problem(runs[r].first, runs[r].second, algo, lvl, zf,
@@ -273,16 +276,20 @@ solver_hip_loader::solver_hip_loader()
             names.push_back(std::string("hip-fdtd-noop") + f);
         }
         for (unsigned int n = 2; n <= 6; n++) {
-            for (const char* f : { "", "-mixed", "-zform" }) {
+            for (const char* f : { "", "-mixed", "-zform", "-mfma",
+                                   "-mfma64" }) {
+                if (n < 4 && std::string(f).rfind("-mfma", 0) == 0) {
+                    continue;
+                }
                 names.push_back("hip-fdtd-" + std::to_string(n) + "lvl-rodr" +
                                 f);
             }
         }
-        for (unsigned int n = 2; n <= 3; n++) {
-            for (const char* f : { "", "-mixed" }) {
-                names.push_back("hip-fdtd-" + std::to_string(n) +
-                                "lvl-reg-cayley" + f);
-            }
+        for (const char* f : { "", "-mixed", "-zform" }) {
+            names.push_back(std::string("hip-fdtd-2lvl-reg-cayley") + f);
+        }
+        for (const char* f : { "", "-mixed" }) {
+            names.push_back(std::string("hip-fdtd-3lvl-reg-cayley") + f);
         }
         for (const auto& nm : names) {
             solver::register_bootstrap(

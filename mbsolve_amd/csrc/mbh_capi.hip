@@ -16,7 +16,7 @@
 #include <string>
 #include <vector>
 
-#include "mbh_tile.hpp"
+#include "mbh_zmfma.hpp"
 
 using namespace mbh;
 
@@ -182,6 +182,115 @@ void bloch2_from_sigma(const double* o, const double* sig, double* rho)
     rho[3] = -g * sig[2];
 }
 
+/* MFMA-form constants of one material (layout zm_geom<N>) from its
+ * z-form blob (layout algo_zform<N>): split FP16 fragments of the scaled
+ * W - I in the kernel's row order, and per-lane rows of theta, a, c and
+ * the diagonal rows of P, o. */
+template<int N>
+void zm_build(const double* zq, unsigned char* out)
+{
+    using Gm = zm_geom<N>;
+    using Z = algo_zform<N>;
+    constexpr int K = Gm::K;
+    std::memset(out, 0, Gm::BYTES);
+    double* hd = reinterpret_cast<double*>(out);
+    if (zq[0] == 0.0) {
+        return;
+    }
+    const double* W = zq + Z::OFF_W;
+    const double* cv = zq + Z::OFF_C;
+    double dmax = 0.0;
+    for (int i = 0; i < K; i++) {
+        for (int k = 0; k < K; k++) {
+            dmax = std::max(dmax, std::fabs(W[i * K + k] - (i == k ? 1.0 : 0.0)));
+        }
+        if (Gm::CFOLD) {
+            dmax = std::max(dmax, std::fabs(cv[i]));
+        }
+    }
+    /* scale 2^s puts max |W - I| near 2^13 (FP16 holds up to 65504) */
+    const int s = (dmax > 0.0) ? 13 - (int)std::ceil(std::log2(dmax)) : 0;
+    const double sc = std::ldexp(1.0, s);
+    double thmax = 0.0;
+    for (int b = 0; b < Gm::B; b++) {
+        thmax = std::max(thmax, std::fabs(zq[Z::OFF_TH + b]));
+    }
+    hd[0] = std::ldexp(1.0, -s);
+    hd[1] = thmax;
+    hd[2] = zq[2];
+    hd[3] = 1.0;
+    _Float16* fr = reinterpret_cast<_Float16*>(out + Gm::FRAG);
+    for (int t = 0; t < Gm::MT; t++) {
+        for (int kc = 0; kc < Gm::KC; kc++) {
+            for (int l = 0; l < 64; l++) {
+                const int i = 16 * t + (l % 16), qq = l / 16;
+                for (int sl = 0; sl < 8; sl++) {
+                    const int u = 8 * kc + sl;
+                    const int col =
+                        (u < Gm::ROWS) ? Gm::row(u / 4, qq, u % 4) : K;
+                    double v = 0.0;
+                    if (i < K && col < K) {
+                        v = (W[i * K + col] - (i == col ? 1.0 : 0.0)) * sc;
+                    } else if (i < K && Gm::CFOLD && u == Gm::ROWS &&
+                               qq == 0) {
+                        v = cv[i] * sc;
+                    }
+                    const _Float16 hi = (_Float16)v;
+                    const _Float16 lo = (_Float16)(v - (double)hi);
+                    fr[(((t * Gm::KC + kc) * 2 + 0) * 64 + l) * 8 + sl] = hi;
+                    fr[(((t * Gm::KC + kc) * 2 + 1) * 64 + l) * 8 + sl] = lo;
+                }
+            }
+        }
+    }
+    double* th = reinterpret_cast<double*>(out + Gm::TH);
+    double* ar = reinterpret_cast<double*>(out + Gm::AR);
+    double* cr = reinterpret_cast<double*>(out + Gm::CR);
+    double* pr = reinterpret_cast<double*>(out + Gm::PR);
+    double* ov = reinterpret_cast<double*>(out + Gm::OV);
+    for (int l = 0; l < 64; l++) {
+        const int qq = l / 16;
+        for (int i = 0; i < Gm::PL; i++) {
+            const int plane = Gm::row(i / 2, qq, 2 * (i % 2)) / 2;
+            th[i * 64 + l] = (plane < Gm::B) ? zq[Z::OFF_TH + plane] : 0.0;
+        }
+        for (int i = 0; i < Gm::ROWS; i++) {
+            const int row = Gm::row(i / 4, qq, i % 4);
+            const bool ok = row < K;
+            ar[i * 64 + l] = ok ? zq[Z::OFF_A + row] : 0.0;
+            cr[i * 64 + l] = ok ? zq[Z::OFF_C + row] : 0.0;
+            for (int kk = 0; kk < Gm::NP; kk++) {
+                const int d = N * (N - 1) + kk;
+                pr[(kk * Gm::ROWS + i) * 64 + l] =
+                    ok ? zq[Z::OFF_P + d * K + row] : 0.0;
+            }
+        }
+    }
+    for (int kk = 0; kk < Gm::NP; kk++) {
+        ov[kk] = zq[Z::OFF_O + N * (N - 1) + kk];
+    }
+}
+
+uint64_t zm_bytes(uint32_t levels)
+{
+    switch (levels) {
+    case 4: return zm_geom<4>::BYTES;
+    case 5: return zm_geom<5>::BYTES;
+    case 6: return zm_geom<6>::BYTES;
+    default: return 0;
+    }
+}
+
+void zm_build_levels(uint32_t levels, const double* zq, unsigned char* out)
+{
+    switch (levels) {
+    case 4: zm_build<4>(zq, out); break;
+    case 5: zm_build<5>(zq, out); break;
+    case 6: zm_build<6>(zq, out); break;
+    default: break;
+    }
+}
+
 } /* namespace */
 
 struct mbh_ctx {
@@ -219,6 +328,15 @@ struct mbh_ctx {
     double* fdtd = nullptr;
     double* qm = nullptr;
     uint64_t qstride = 0;
+    void* frag = nullptr; /* MFMA form constants */
+    uint64_t frag_stride = 0;
+    /* MFMA form: tiles of one material run the MFMA kernel (list a), the
+     * others the VALU z-form kernel of the same state layout (list b) */
+    bool split = false;
+    uint32_t* tiles_a = nullptr;
+    uint32_t* tiles_b = nullptr;
+    uint64_t n_tiles_a = 0, n_tiles_b = 0;
+    tile_launch_fn launch_b = nullptr;
     dev_source* src = nullptr;
     uint32_t* tile_src = nullptr;
     dev_record* rec = nullptr;
@@ -394,8 +512,15 @@ int validate(const mbh_problem* p, uint32_t r, const mbh_problem* p0,
                         (unsigned long long)qs);
         }
     }
+    const bool mfma = (form == MBH_FORM_MFMA || form == MBH_FORM_MFMA64);
+    if (mfma && (p->algo != MBH_ALGO_CVR_RODR || p->num_levels < 4)) {
+        return fail(nullptr, MBH_ERR_UNSUPPORTED,
+                    "run %u: the MFMA form is for Rodrigues with 4 to 6 "
+                    "levels",
+                    r);
+    }
     if (p->algo == MBH_ALGO_CVR_RODR &&
-        (form == MBH_FORM_MIXED || form == MBH_FORM_ZFORM64)) {
+        (form == MBH_FORM_MIXED || form == MBH_FORM_ZFORM64 || mfma)) {
         if (!p->zform_consts ||
             p->zform_stride != MBH_ZFORM_STRIDE(p->num_levels)) {
             return fail(nullptr, MBH_ERR_INVALID,
@@ -453,11 +578,11 @@ int validate(const mbh_problem* p, uint32_t r, const mbh_problem* p0,
 const kernel_entry* find_kernel(uint32_t algo, uint32_t levels, uint32_t form,
                                 uint32_t C)
 {
-    const kernel_entry* tabs[3] = { k_table_basic, k_table_rodr,
-                                    k_table_zform };
-    const uint32_t ns[3] = { k_table_basic_n, k_table_rodr_n,
-                             k_table_zform_n };
-    for (int t = 0; t < 3; t++) {
+    const kernel_entry* tabs[4] = { k_table_basic, k_table_rodr,
+                                    k_table_zform, k_table_zm };
+    const uint32_t ns[4] = { k_table_basic_n, k_table_rodr_n,
+                             k_table_zform_n, k_table_zm_n };
+    for (int t = 0; t < 4; t++) {
         for (uint32_t i = 0; i < ns[t]; i++) {
             const kernel_entry& k = tabs[t][i];
             if (k.algo == algo && k.levels == levels && k.form == form &&
@@ -553,8 +678,37 @@ int build_tile_tables(mbh_ctx* ctx, bool with_records)
     release(ctx->tile_src);
     release(ctx->tile_rec_ptr);
     release(ctx->tile_rec_idx);
+    release(ctx->tiles_a);
+    release(ctx->tiles_b);
     ctx->tile_src = ctx->tile_rec_ptr = ctx->tile_rec_idx = nullptr;
+    ctx->tiles_a = ctx->tiles_b = nullptr;
+    ctx->n_tiles_a = ctx->n_tiles_b = 0;
     int rc;
+    if (ctx->split) {
+        /* a tile is uniform when all its 64 cells lie in the grid and have
+         * the same material */
+        std::vector<uint32_t> la, lb;
+        for (uint64_t t = 0; t < nt; t++) {
+            const int64_t lo = (int64_t)(t * ctx->owned) - ctx->halo;
+            const int64_t hi = lo + 64 * (int64_t)ctx->C;
+            bool uni = lo >= 0 && hi <= (int64_t)ctx->G;
+            if (uni) {
+                const int32_t m0 = ctx->mat_h[lo];
+                for (int64_t g = lo; g < hi && uni; g++) {
+                    uni = ctx->mat_h[g] == m0 && m0 >= 0;
+                }
+            }
+            (uni ? la : lb).push_back((uint32_t)t);
+        }
+        ctx->n_tiles_a = la.size();
+        ctx->n_tiles_b = lb.size();
+        if ((rc = upload(ctx, &ctx->tiles_a, la.data(), la.size()))) {
+            return rc;
+        }
+        if ((rc = upload(ctx, &ctx->tiles_b, lb.data(), lb.size()))) {
+            return rc;
+        }
+    }
     if ((rc = upload(ctx, &ctx->tile_src, ts.data(), ts.size()))) {
         return rc;
     }
@@ -566,6 +720,7 @@ int build_tile_tables(mbh_ctx* ctx, bool with_records)
 
 KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
                 uint32_t nsteps);
+hipError_t launch_block(const mbh_ctx* ctx, KArgs k);
 
 /*
  * Autotuner: times every admissible (cells per lane, temporal block) pair
@@ -636,7 +791,7 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
         KArgs k = make_args(ctx, ctx->cur, 0, steps);
         /* one warm-up launch, timed */
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        HIP_TRY(ctx, ctx->launch(k, ctx->num_tiles, ctx->stream));
+        HIP_TRY(ctx, launch_block(ctx, k));
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
         float ms0 = 0.0f;
@@ -644,7 +799,7 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
         const int reps = (ms0 > 20.0f) ? 1 : 4;
         HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
         for (int i = 0; i < reps; i++) {
-            HIP_TRY(ctx, ctx->launch(k, ctx->num_tiles, ctx->stream));
+            HIP_TRY(ctx, launch_block(ctx, k));
         }
         HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
         HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
@@ -695,7 +850,30 @@ KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
     k.nsteps = nsteps;
     k.halo = ctx->halo;
     k.owned = ctx->owned;
+    k.frag = ctx->frag;
+    k.frag_stride = ctx->frag_stride;
     return k;
+}
+
+/* One temporal block of all tiles: one launch, or with split tile lists
+ * the MFMA kernel on the uniform tiles and the VALU kernel on the rest
+ * (disjoint owned cells, same input and output buffers). */
+hipError_t launch_block(const mbh_ctx* ctx, KArgs k)
+{
+    if (!ctx->split) {
+        k.tile_list = nullptr;
+        return ctx->launch(k, ctx->num_tiles, ctx->stream);
+    }
+    hipError_t e = hipSuccess;
+    if (ctx->n_tiles_a) {
+        k.tile_list = ctx->tiles_a;
+        e = ctx->launch(k, ctx->n_tiles_a, ctx->stream);
+    }
+    if (e == hipSuccess && ctx->n_tiles_b) {
+        k.tile_list = ctx->tiles_b;
+        e = ctx->launch_b(k, ctx->n_tiles_b, ctx->stream);
+    }
+    return e;
 }
 
 int build_graph(mbh_ctx* ctx, int start_buf)
@@ -707,7 +885,7 @@ int build_graph(mbh_ctx* ctx, int start_buf)
     hipError_t le = hipSuccess;
     for (uint32_t j = 0; j < ctx->graph_launches && le == hipSuccess; j++) {
         KArgs k = make_args(ctx, buf, (uint64_t)j * ctx->tb, ctx->tb);
-        le = ctx->launch(k, ctx->num_tiles, ctx->stream);
+        le = launch_block(ctx, k);
         buf = 1 - buf;
     }
     if (le == hipSuccess) {
@@ -781,7 +959,7 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     if (opt_in) {
         opt = *opt_in;
     }
-    if (opt.form > MBH_FORM_ZFORM64) {
+    if (opt.form > MBH_FORM_MFMA64) {
         return fail(nullptr, MBH_ERR_INVALID, "unknown form %u", opt.form);
     }
     for (uint32_t r = 0; r < num_runs; r++) {
@@ -815,9 +993,10 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     c->n_t = p0.num_timesteps;
     c->dref = ref_density_stride(c->algo, c->levels);
     c->DC = c->dref;
-    c->fp32 = (opt.form == MBH_FORM_MIXED);
     c->zform = (c->algo == MBH_ALGO_CVR_RODR) &&
-        (opt.form == MBH_FORM_MIXED || opt.form == MBH_FORM_ZFORM64);
+        (opt.form == MBH_FORM_MIXED || opt.form == MBH_FORM_ZFORM64 ||
+         opt.form == MBH_FORM_MFMA || opt.form == MBH_FORM_MFMA64);
+    c->fp32 = (opt.form == MBH_FORM_MIXED || opt.form == MBH_FORM_MFMA);
     c->bloch = (c->algo == MBH_ALGO_REG_CAYLEY) && c->levels == 2 &&
         (opt.form == MBH_FORM_MIXED || opt.form == MBH_FORM_ZFORM64);
     if (c->bloch) {
@@ -920,6 +1099,18 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     RC(upload(c, &c->hzero, hz.data(), hz.size()));
     RC(upload(c, &c->fdtd, fdtd.data(), fdtd.size()));
     RC(upload(c, &c->qm, qm.data(), qm.size()));
+    if (c->form == MBH_FORM_MFMA || c->form == MBH_FORM_MFMA64) {
+        const uint64_t nm = qm.size() / std::max<uint64_t>(1, c->qstride);
+        c->frag_stride = zm_bytes(c->levels);
+        std::vector<unsigned char> fr(nm * c->frag_stride);
+        for (uint64_t mi = 0; mi < nm; mi++) {
+            zm_build_levels(c->levels, &qm[mi * c->qstride],
+                            &fr[mi * c->frag_stride]);
+        }
+        unsigned char* dfr = nullptr;
+        RC(upload(c, &dfr, fr.data(), fr.size()));
+        c->frag = dfr;
+    }
 
     /* state buffers: e, h, p [G], density [DC][dstride] */
     for (int b = 0; b < 2; b++) {
@@ -1071,6 +1262,21 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     HIP_TRY(c, hipMemset(c->n_base, 0, sizeof(uint64_t)));
     HIP_TRY(c, hipDeviceSynchronize());
 
+    /* MFMA form: uniform tiles on the matrix cores, mixed tiles on the
+     * VALU z-form kernel of the same state layout */
+    if (c->form == MBH_FORM_MFMA || c->form == MBH_FORM_MFMA64) {
+        const kernel_entry* kb = find_kernel(
+            c->algo, c->levels,
+            c->fp32 ? MBH_FORM_MIXED : MBH_FORM_ZFORM64, 1);
+        if (!kb) {
+            return fail(nullptr, MBH_ERR_UNSUPPORTED,
+                        "no z-form kernel for the mixed tiles");
+        }
+        c->split = true;
+        c->launch_b = kb->fn;
+        c->mat_h = mat;
+    }
+
     /* launch geometry: measured, then the final per-tile tables */
     RC(autotune(c, opt.temporal_block, opt.tile_cells));
     RC(build_tile_tables(c, true));
@@ -1119,7 +1325,7 @@ int mbh_run(mbh_ctx* ctx, uint64_t n_begin, uint64_t n_end)
         }
         const uint32_t steps = (uint32_t)std::min<uint64_t>(ctx->tb, left);
         KArgs k = make_args(ctx, ctx->cur, n - base, steps);
-        HIP_TRY(ctx, ctx->launch(k, ctx->num_tiles, ctx->stream));
+        HIP_TRY(ctx, launch_block(ctx, k));
         ctx->cur = 1 - ctx->cur;
         n += steps;
         launches++;

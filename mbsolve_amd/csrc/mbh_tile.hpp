@@ -78,6 +78,9 @@ struct KArgs {
     uint32_t halo;
     uint32_t owned;
     uint32_t pad;
+    const void* frag;     /* MFMA form: [nmat][frag_stride bytes] */
+    uint64_t frag_stride;
+    const uint32_t* tile_list; /* tiles of this launch (NULL: blockIdx) */
 };
 
 template<typename ST>
@@ -201,7 +204,7 @@ __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
     __shared__ double s_src[TILE_SOURCES][MAX_BLOCK];
 
     const int lane = threadIdx.x;
-    const uint64_t t = blockIdx.x;
+    const uint64_t t = k.tile_list ? k.tile_list[blockIdx.x] : blockIdx.x;
     const int64_t own_lo = (int64_t)(t * k.owned);
     const int64_t own_hi =
         ((own_lo + (int64_t)k.owned) < (int64_t)k.Gown) ? own_lo + k.owned
@@ -447,6 +450,8 @@ extern const kernel_entry k_table_rodr[];
 extern const uint32_t k_table_rodr_n;
 extern const kernel_entry k_table_zform[];
 extern const uint32_t k_table_zform_n;
+extern const kernel_entry k_table_zm[];
+extern const uint32_t k_table_zm_n;
 extern const zconv_entry z_table[];
 extern const uint32_t z_table_n;
 

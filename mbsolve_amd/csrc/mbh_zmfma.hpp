@@ -1,0 +1,509 @@
+/*
+ * Rodrigues z-form tile kernel with the generator product on the matrix
+ * cores (gfx950 MFMA), for N = 4..6 levels (K = N^2 - 1 = 15, 24, 35).
+ *
+ * One step of the z-form (lindblad_cvr_rodr::update,
+ * algo_lindblad_cvr_rodr.hpp:234-268, restated in the rotated basis):
+ *   r = R(e) y,  p = a . r + b,  y' = r + c + (W - I) r.
+ * The affine part and the rotation stay in FP64; the product (W - I) r,
+ * a K x K matrix shared by all cells of a material times the K x 64 block
+ * of the tile's rotated states, runs as v_mfma_f32_16x16x32_f16 with both
+ * operands split into FP16 hi + lo parts (A_hi r_hi + A_hi r_lo +
+ * A_lo r_hi, FP32 accumulation).  (W - I) is scaled by a power of two so
+ * that its entries use the FP16 range; the products are scaled back in
+ * the FP64 accumulation.  W - I is O(rate * dt), so the split's relative
+ * error of about 2^-21 enters the state at about 1e-10 per step.
+ *
+ * Layout.  One wavefront owns a tile of 64 cells.  The FDTD part keeps the
+ * tile_kernel layout (lane l = cell l).  The density state of the 64 cells
+ * is four 16-cell blocks n = 0..3 in the MFMA accumulator layout: lane
+ * (q, j) = (l / 16, l % 16) holds rows 16 t + 4 q + rr (t < MT, rr < 4) of
+ * cell 16 n + j.  The MFMA output of M-tile t lands on exactly these rows
+ * of the same lane, and the next step's B operand is filled from the
+ * lane's own rows: K-slot s of chunk kc of lane group q stands for row
+ * 16 (u / 4) + 4 q + u % 4 with u = 8 kc + s, and the host builds the A
+ * fragments with the same row order, so no data crosses lanes between
+ * steps.  Rotation planes pair rows (2 b, 2 b + 1), which lie in one lane.
+ */
+#ifndef MBH_ZMFMA_HPP
+#define MBH_ZMFMA_HPP
+
+#include "mbh_tile.hpp"
+
+namespace mbh {
+
+typedef _Float16 zm_h8 __attribute__((ext_vector_type(8)));
+typedef float zm_f4 __attribute__((ext_vector_type(4)));
+
+/* geometry and per-material constant layout (bytes) of the MFMA form */
+template<int N>
+struct zm_geom {
+    static constexpr int K = N * N - 1;
+    static constexpr int B = K / 2;
+    static constexpr int MT = (K + 15) / 16; /* M-tiles of 16 rows */
+    static constexpr int ROWS = 4 * MT;      /* rows per lane and block */
+    static constexpr int KC = (ROWS + 7) / 8; /* K chunks of 32 */
+    static constexpr int PL = 2 * MT;        /* rotation planes per lane */
+    static constexpr int NP = N - 1;         /* diagonal generators */
+    static constexpr int NB = 4;             /* 16-cell blocks per tile */
+    /* with a free K slot (u = ROWS) the constant c rides in the product:
+     * B holds 1 there in lane group 0, A holds c in that column */
+    static constexpr bool CFOLD = (ROWS % 8) != 0;
+    /* header: [0] 2^-s (scale of the product) [1] max theta_dt
+     *         [2] b (p offset) [3] has_qm */
+    static constexpr uint64_t FRAG = 32;     /* zm_h8 [MT][KC][2][64] */
+    static constexpr uint64_t TH = FRAG + (uint64_t)MT * KC * 2 * 64 * 16;
+    static constexpr uint64_t AR = TH + (uint64_t)PL * 64 * 8;   /* a rows */
+    static constexpr uint64_t CR = AR + (uint64_t)ROWS * 64 * 8; /* c rows */
+    static constexpr uint64_t PR = CR + (uint64_t)ROWS * 64 * 8; /* P rows */
+    static constexpr uint64_t OV = PR + (uint64_t)NP * ROWS * 64 * 8; /* o */
+    static constexpr uint64_t BYTES = (OV + NP * 8 + 255) & ~uint64_t(255);
+
+    __host__ __device__ static constexpr int row(int t, int q, int rr)
+    {
+        return 16 * t + 4 * q + rr;
+    }
+};
+
+/* sin and cos of a small angle.  TIER 0: |x| <= 2^-10, Taylor to x^5 /
+ * x^4 (relative truncation error below 2e-21); TIER 1: |x| <= 0.25,
+ * Taylor to x^13 / x^14; TIER 2: library. */
+template<int TIER>
+__device__ __forceinline__ void tier_sincos(double x, double& s, double& c)
+{
+    if constexpr (TIER == 0) {
+        const double x2 = x * x;
+        s = fma(x * x2, fma(x2, 8.3333333333333333333e-03,
+                            -1.6666666666666666667e-01), x);
+        c = fma(x2, fma(x2, 4.1666666666666666667e-02, -0.5), 1.0);
+    } else if constexpr (TIER == 1) {
+        taylor_sincos(x, s, c);
+    } else {
+        sincos(x, &s, &c);
+    }
+}
+
+/* Lane (q, j) holds v[n] for blocks n = 0..3; returns sum over the four
+ * lane groups of v[q] (block n = q, i.e. the cell of this lane in the
+ * one-cell-per-lane layout).  Two exchange rounds (xor 32, xor 16). */
+__device__ __forceinline__ double reduce_scatter4(const double* v, int q)
+{
+    const bool b = (q & 2) != 0, c = (q & 1) != 0;
+    const double keep0 = b ? v[2] : v[0], keep1 = b ? v[3] : v[1];
+    const double send0 = b ? v[0] : v[2], send1 = b ? v[1] : v[3];
+    const double s0 = keep0 + __shfl_xor(send0, 32);
+    const double s1 = keep1 + __shfl_xor(send1, 32);
+    const double keep = c ? s1 : s0, send = c ? s0 : s1;
+    return keep + __shfl_xor(send, 16);
+}
+
+/* per-material constants of one lane */
+template<int N>
+struct zm_consts {
+    using Gm = zm_geom<N>;
+    zm_h8 ahi[Gm::MT][Gm::KC], alo[Gm::MT][Gm::KC];
+    double th[Gm::PL], ar[Gm::ROWS], cr[Gm::ROWS];
+    double scale, thmax, b;
+
+    __device__ void load(const char* base, int lane)
+    {
+        const zm_h8* f = reinterpret_cast<const zm_h8*>(base + Gm::FRAG);
+#pragma unroll
+        for (int t = 0; t < Gm::MT; t++) {
+#pragma unroll
+            for (int kc = 0; kc < Gm::KC; kc++) {
+                ahi[t][kc] = f[((t * Gm::KC + kc) * 2 + 0) * 64 + lane];
+                alo[t][kc] = f[((t * Gm::KC + kc) * 2 + 1) * 64 + lane];
+            }
+        }
+        const double* th_ = reinterpret_cast<const double*>(base + Gm::TH);
+        const double* ar_ = reinterpret_cast<const double*>(base + Gm::AR);
+        const double* cr_ = reinterpret_cast<const double*>(base + Gm::CR);
+#pragma unroll
+        for (int i = 0; i < Gm::PL; i++) {
+            th[i] = th_[i * 64 + lane];
+        }
+#pragma unroll
+        for (int i = 0; i < Gm::ROWS; i++) {
+            ar[i] = ar_[i * 64 + lane];
+            cr[i] = Gm::CFOLD ? 0.0 : cr_[i * 64 + lane];
+        }
+        const double* hd = reinterpret_cast<const double*>(base);
+        scale = hd[0];
+        thmax = hd[1];
+        b = hd[2];
+    }
+};
+
+/* One density step of the tile's four 16-cell blocks.  Phase 1 (VALU,
+ * FP64): rotate each block's rows in place (Y := r), this lane's partial
+ * a . r into pp, FP16 hi/lo B operands.  Phase 2: the split products into
+ * one FP32 accumulator per (block, M-tile), issued chunk by chunk across
+ * the twelve independent accumulators.  Phase 3: y' = r + 2^-s acc (+ c
+ * when c is not carried in the product). */
+template<int N, int TIER>
+__device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
+                                        const double* eb, float one_q0,
+                                        double (*Y)[zm_geom<N>::ROWS],
+                                        double* pp)
+{
+    using Gm = zm_geom<N>;
+    zm_h8 bh[Gm::NB][Gm::KC], bl[Gm::NB][Gm::KC];
+#pragma unroll
+    for (int n = 0; n < Gm::NB; n++) {
+#pragma unroll
+        for (int t = 0; t < Gm::MT; t++) {
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                double sn, cs;
+                tier_sincos<TIER>(K_.th[2 * t + u] * eb[n], sn, cs);
+                const double y0 = Y[n][4 * t + 2 * u];
+                const double y1 = Y[n][4 * t + 2 * u + 1];
+                Y[n][4 * t + 2 * u] = fma(cs, y0, -(sn * y1));
+                Y[n][4 * t + 2 * u + 1] = fma(sn, y0, cs * y1);
+            }
+        }
+        double pa = 0.0;
+#pragma unroll
+        for (int i = 0; i < Gm::ROWS; i++) {
+            pa = fma(K_.ar[i], Y[n][i], pa);
+        }
+        pp[n] = pa;
+#pragma unroll
+        for (int kc = 0; kc < Gm::KC; kc++) {
+#pragma unroll
+            for (int sl = 0; sl < 8; sl++) {
+                const int u = 8 * kc + sl;
+                const float v = (u < Gm::ROWS)
+                    ? (float)Y[n][u]
+                    : ((Gm::CFOLD && u == Gm::ROWS) ? one_q0 : 0.0f);
+                const _Float16 hi = (_Float16)v;
+                bh[n][kc][sl] = hi;
+                bl[n][kc][sl] = (_Float16)(v - (float)hi);
+            }
+        }
+    }
+    zm_f4 acc[Gm::NB][Gm::MT];
+#pragma unroll
+    for (int n = 0; n < Gm::NB; n++) {
+#pragma unroll
+        for (int t = 0; t < Gm::MT; t++) {
+            acc[n][t] = zm_f4{ 0.0f, 0.0f, 0.0f, 0.0f };
+        }
+    }
+#pragma unroll
+    for (int kc = 0; kc < Gm::KC; kc++) {
+#pragma unroll
+        for (int n = 0; n < Gm::NB; n++) {
+#pragma unroll
+            for (int t = 0; t < Gm::MT; t++) {
+                acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                    K_.ahi[t][kc], bh[n][kc], acc[n][t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < Gm::NB; n++) {
+#pragma unroll
+            for (int t = 0; t < Gm::MT; t++) {
+                acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                    K_.ahi[t][kc], bl[n][kc], acc[n][t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int n = 0; n < Gm::NB; n++) {
+#pragma unroll
+            for (int t = 0; t < Gm::MT; t++) {
+                acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                    K_.alo[t][kc], bh[n][kc], acc[n][t], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < Gm::NB; n++) {
+#pragma unroll
+        for (int t = 0; t < Gm::MT; t++) {
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int i = 4 * t + rr;
+                Y[n][i] = Gm::CFOLD
+                    ? fma((double)acc[n][t][rr], K_.scale, Y[n][i])
+                    : fma((double)acc[n][t][rr], K_.scale,
+                          Y[n][i] + K_.cr[i]);
+            }
+        }
+    }
+}
+
+template<int N, typename ST>
+__global__ __launch_bounds__(64) void zm_kernel(KArgs k)
+{
+    using Gm = zm_geom<N>;
+    __shared__ double s_src[TILE_SOURCES][MAX_BLOCK];
+
+    const int lane = threadIdx.x;
+    const int q = lane >> 4, j = lane & 15;
+    const float one_q0 = (q == 0) ? 1.0f : 0.0f;
+    const uint64_t t = k.tile_list ? k.tile_list[blockIdx.x] : blockIdx.x;
+    const int64_t own_lo = (int64_t)(t * k.owned);
+    const int64_t own_hi =
+        ((own_lo + (int64_t)k.owned) < (int64_t)k.Gown) ? own_lo + k.owned
+                                                         : (int64_t)k.Gown;
+    const int64_t g0 = own_lo - (int64_t)k.halo;
+    const int64_t gl = g0 + lane;
+    const uint64_t n0 = *as_const(k.n_base) + k.n_off;
+    const uint32_t nsteps = k.nsteps;
+
+    /* FDTD state, one cell per lane (folded constants as in tile_kernel) */
+    const bool in = (gl >= 0) && (gl < (int64_t)k.G);
+    const int32_t m = in ? k.mat[gl] : -1;
+    const bool hz = in ? (k.hzero[gl] != 0) : true;
+    double e = in ? ld<ST>(k.e_in, gl) : 0.0;
+    double h = in ? ld<ST>(k.h_in, gl) : 0.0;
+    double p = in ? ld<ST>(k.p_in, gl) : 0.0;
+    double fa = 0.0, fg = 0.0, fx = 0.0, fc = 0.0;
+    if (m >= 0) {
+        const double* f = k.fdtd + (uint64_t)m * FDTD_STRIDE;
+        fa = f[0];
+        fg = -(f[1] * f[3]);
+        fx = f[1] * f[4];
+        fc = hz ? 0.0 : f[2];
+    }
+    const double hm = hz ? 0.0 : 1.0;
+
+    /* density state, MFMA layout */
+    double Y[Gm::NB][Gm::ROWS];
+    int32_t mb[Gm::NB];
+#pragma unroll
+    for (int n = 0; n < Gm::NB; n++) {
+        const int64_t gn = g0 + 16 * n + j;
+        const bool inn = (gn >= 0) && (gn < (int64_t)k.G);
+        mb[n] = __shfl(m, 16 * n + j);
+#pragma unroll
+        for (int tt = 0; tt < Gm::MT; tt++) {
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int row = Gm::row(tt, q, rr);
+                Y[n][4 * tt + rr] = (inn && row < Gm::K)
+                    ? ld<ST>(k.d_in, (uint64_t)row * k.dstride + gn)
+                    : 0.0;
+            }
+        }
+    }
+
+    /* material constants: one uniform material (the common case) is
+     * loaded once; mixed tiles go through a per-material loop */
+    const int32_t m0 = __builtin_amdgcn_readfirstlane(m);
+    const bool uniform = __all(m == m0);
+    const char* fbase = static_cast<const char*>(k.frag);
+    const bool q0_active = uniform && m0 >= 0 &&
+        as_const(reinterpret_cast<const double*>(
+            fbase + (uint64_t)m0 * k.frag_stride))[3] != 0.0;
+    zm_consts<N> KC0;
+    if (q0_active) {
+        KC0.load(fbase + (uint64_t)m0 * k.frag_stride, lane);
+    }
+    bool my_qm = false; /* this lane's cell has a density */
+    if (m >= 0) {
+        my_qm = reinterpret_cast<const double*>(
+                    fbase + (uint64_t)m * k.frag_stride)[3] != 0.0;
+    }
+
+    /* sources of this tile */
+    const uint32_t s_lo = as_const(k.tile_src)[2 * t];
+    const uint32_t s_hi = as_const(k.tile_src)[2 * t + 1];
+    const uint32_t nsrc = s_hi - s_lo;
+    const bool src_staged = nsrc <= TILE_SOURCES && nsteps <= MAX_BLOCK;
+    if (src_staged && nsrc > 0) {
+        for (uint32_t jj = 0; jj < nsrc; jj++) {
+            const double* data = as_const(k.src)[s_lo + jj].data;
+            s_src[jj][lane] = (lane < (int)nsteps) ? data[n0 + lane] : 0.0;
+        }
+        __syncthreads();
+    }
+
+    /* record steps of this block */
+    const uint32_t r_lo = as_const(k.tile_rec_ptr)[t];
+    const uint32_t r_hi = as_const(k.tile_rec_ptr)[t + 1];
+    uint64_t rec_steps = 0;
+    bool rec_density = false;
+    for (uint32_t ri = r_lo; ri < r_hi; ri++) {
+        const uint32_t idx = as_const(k.tile_rec_idx)[ri];
+        const double ratio = as_const(k.rec)[idx].ratio;
+        const uint64_t rows = as_const(k.rec)[idx].rows;
+        const uint32_t type = as_const(k.rec)[idx].type;
+        rec_steps |= __ballot(lane < (int)nsteps &&
+                              record_now(ratio, rows, n0 + lane));
+        rec_density = rec_density || type == MBH_REC_DENSITY ||
+            type == MBH_REC_INVERSION;
+    }
+
+    for (uint32_t s = 0; s < nsteps; s++) {
+        const uint64_t n = n0 + s;
+
+        /* E update */
+        const double h_right = from_right(h);
+        double en = fma(fx, h_right - h, fma(fg, p, fa * e));
+        /* sources, in list order */
+        if (nsrc > 0) {
+            for (uint32_t jj = 0; jj < nsrc; jj++) {
+                const uint64_t gx = as_const(k.src)[s_lo + jj].gx;
+                const uint32_t type = as_const(k.src)[s_lo + jj].type;
+                const double v = src_staged
+                    ? s_src[jj][s]
+                    : as_const(k.src)[s_lo + jj].data[n];
+                if ((int64_t)gx == gl) {
+                    en = (type == MBH_SRC_HARD) ? v : en + v;
+                }
+            }
+        }
+        /* H update and boundary condition */
+        const double e_left = from_left(en);
+        h = fma(fc, en - e_left, hm * h);
+        e = en;
+
+        /* density update */
+        double eb[Gm::NB];
+#pragma unroll
+        for (int nb = 0; nb < Gm::NB; nb++) {
+            eb[nb] = __shfl(e, 16 * nb + j);
+        }
+        if (q0_active) {
+            double emax = fabs(eb[0]);
+#pragma unroll
+            for (int nb = 1; nb < Gm::NB; nb++) {
+                emax = fmax(emax, fabs(eb[nb]));
+            }
+            const double xmax = KC0.thmax * emax;
+            const int tier = __all(xmax <= 0x1p-10) ? 0
+                : (__all(xmax <= TAYLOR_MAX) ? 1 : 2);
+            double pp[Gm::NB];
+            if (tier == 0) {
+                zm_step<N, 0>(KC0, eb, one_q0, Y, pp);
+            } else if (tier == 1) {
+                zm_step<N, 1>(KC0, eb, one_q0, Y, pp);
+            } else {
+                zm_step<N, 2>(KC0, eb, one_q0, Y, pp);
+            }
+            p = reduce_scatter4(pp, q) + KC0.b;
+        }
+        /* tiles with more than one material (or none) go to the VALU
+         * z-form kernel, which shares this state layout */
+
+        /* record copy */
+        if ((rec_steps >> s) & 1ull) {
+            /* diagonal coherence-vector components of this lane's cell:
+             * d_k = o_k + sum_i P[k][i] y_i, k = N(N-1) .. K-1 */
+            double dd[Gm::K];
+#pragma unroll
+            for (int kk = 0; kk < Gm::K; kk++) {
+                dd[kk] = 0.0;
+            }
+            if (rec_density) {
+#pragma unroll 1
+                for (int kk = 0; kk < Gm::NP; kk++) {
+                    double part[Gm::NB];
+#pragma unroll
+                    for (int nb = 0; nb < Gm::NB; nb++) {
+                        double acc = 0.0;
+                        if (mb[nb] >= 0) {
+                            const double* pr = reinterpret_cast<const double*>(
+                                fbase + (uint64_t)mb[nb] * k.frag_stride +
+                                Gm::PR);
+#pragma unroll
+                            for (int i = 0; i < Gm::ROWS; i++) {
+                                acc = fma(pr[(kk * Gm::ROWS + i) * 64 + lane],
+                                          Y[nb][i], acc);
+                            }
+                        }
+                        part[nb] = acc;
+                    }
+                    double v = reduce_scatter4(part, q);
+                    if (my_qm) {
+                        v += reinterpret_cast<const double*>(
+                            fbase + (uint64_t)m * k.frag_stride +
+                            Gm::OV)[kk];
+                    } else {
+                        v = 0.0;
+                    }
+#pragma unroll
+                    for (int z = 0; z < Gm::NP; z++) {
+                        if (z == kk) {
+                            dd[N * (N - 1) + z] = v;
+                        }
+                    }
+                }
+            }
+            for (uint32_t ri = r_lo; ri < r_hi; ri++) {
+                const dev_record R =
+                    load_record(k.rec, as_const(k.tile_rec_idx)[ri]);
+                if (!record_now(R.ratio, R.rows, n)) {
+                    continue;
+                }
+                if (R.type == MBH_REC_DENSITY && R.row_idx != R.col_idx) {
+                    continue; /* coherences are not recorded */
+                }
+                const uint64_t row = (uint64_t)floor(R.ratio * (double)n);
+                if (gl >= own_lo && gl < own_hi &&
+                    gl >= (int64_t)R.gbegin &&
+                    gl < (int64_t)(R.gbegin + R.cols)) {
+                    double v;
+                    switch (R.type) {
+                    case MBH_REC_ELECTRIC:
+                        v = e;
+                        break;
+                    case MBH_REC_POLAR_DT:
+                        v = p;
+                        break;
+                    case MBH_REC_MAGNETIC:
+                        v = h;
+                        break;
+                    case MBH_REC_INVERSION:
+                        v = my_qm ? dd[N * (N - 1)] : 0.0;
+                        break;
+                    default:
+                        v = my_qm ? cv_population<N>(dd, R.row_idx) : 0.0;
+                        break;
+                    }
+                    R.dst[row * R.cols + (uint64_t)(gl - (int64_t)R.gbegin)] =
+                        v;
+                }
+            }
+        }
+    }
+
+    /* write back the owned cells */
+    if (gl >= own_lo && gl < own_hi) {
+        st<ST>(k.e_out, gl, e);
+        st<ST>(k.h_out, gl, h);
+        st<ST>(k.p_out, gl, p);
+    }
+#pragma unroll
+    for (int n = 0; n < Gm::NB; n++) {
+        const int64_t gn = g0 + 16 * n + j;
+        if (gn >= own_lo && gn < own_hi) {
+#pragma unroll
+            for (int tt = 0; tt < Gm::MT; tt++) {
+#pragma unroll
+                for (int rr = 0; rr < 4; rr++) {
+                    const int row = Gm::row(tt, q, rr);
+                    if (row < Gm::K) {
+                        st<ST>(k.d_out, (uint64_t)row * k.dstride + gn,
+                               Y[n][4 * tt + rr]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template<int N, typename ST>
+hipError_t launch_zm(const KArgs& k, uint64_t num_tiles, hipStream_t stream)
+{
+    hipLaunchKernelGGL((zm_kernel<N, ST>), dim3((unsigned)num_tiles),
+                       dim3(64), 0, stream, k);
+    return hipGetLastError();
+}
+
+} /* namespace mbh */
+
+#endif

@@ -13,7 +13,7 @@ from conftest import rel_err
 
 pytestmark = pytest.mark.gpu
 
-FP64_TOL = 1e-12   # north-star bound for the FP64 form
+FP64_TOL = 1e-12   # bound for the FP64 form against the oracle
 MIXED_TOL = 1e-5   # bound for the FP32-storage form
 
 
@@ -127,3 +127,58 @@ def test_capi_direct(gpu):
     for k in ("e", "h", "p", "d"):
         assert np.array_equal(st[k], rs[k]), k
     ctx.close()
+
+
+MFMA_TOL = 1e-6  # split-FP16 generator product, short runs
+
+
+def _qcl6_with_populations():
+    spec = setups.qcl6(num_gridpoints=2000, num_timesteps=2500,
+                       num_records=10, cw_ampl=3e6)
+    for i in (1, 3, 6):
+        spec["records"].append({"name": "d%d%d" % (i, i), "type": 3,
+                                "row": i, "col": i, "interval": 0.0,
+                                "position": 1.0e-6})
+    return spec
+
+
+@pytest.mark.parametrize("form", ["mfma", "mfma64"])
+@pytest.mark.parametrize("make", [
+    _qcl6_with_populations,
+    lambda: setups.marskar2011(4, num_gridpoints=700, endtime=0.2e-12),
+    lambda: setups.marskar2011(5, num_gridpoints=400, endtime=0.1e-12),
+], ids=["qcl6", "marskar4", "marskar5"])
+def test_mfma_forms_vs_oracle(gpu, form, make):
+    """Rodrigues z-form with the generator product on MFMA (split FP16,
+    FP32 accumulation) against the oracle: fields relative to their
+    maximum, populations on the trace-1 scale."""
+    spec = make()
+    sol, recs = _run_hip(spec, form)
+    ref = _oracle_records(sol)
+    for r, a, b in zip(sol.get_results(), recs, ref):
+        if r.get_name().startswith(("d", "inv")):
+            err = float(np.max(np.abs(a - b)))
+        else:
+            err = rel_err(a, b)
+        assert err <= MFMA_TOL, (r.get_name(), err)
+
+
+def test_mfma_batch_mixed_tiles(gpu):
+    """Three six-level runs of uneven length in one context: tiles that
+    straddle run boundaries take the VALU z-form kernel, the rest the MFMA
+    kernel; every run matches its own oracle."""
+    specs = [setups.qcl6(num_gridpoints=n, num_timesteps=1500, seed=s,
+                         num_records=6, cw_ampl=3e6, name="q%d" % s)
+             for s, n in ((1, 700), (2, 1100), (3, 333))]
+    mb.material.clear_library()
+    runs = [setups.build(s, clear_library=False) for s in specs]
+    print("case mfma batch", flush=True)
+    batch = mb.solver_hip_batch("hip-fdtd-6lvl-rodr-mfma", runs)
+    batch.run()
+    for r in range(len(specs)):
+        ref = oracle.OracleRun(batch.problem(r)).run().records()
+        for x, b in zip(batch.get_results(r), ref):
+            a = x.get_data_real()
+            err = (float(np.max(np.abs(a - b)))
+                   if x.get_name().startswith("inv") else rel_err(a, b))
+            assert err <= MFMA_TOL, (r, x.get_name(), err)
