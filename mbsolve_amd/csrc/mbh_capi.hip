@@ -330,13 +330,12 @@ struct mbh_ctx {
     uint64_t qstride = 0;
     void* frag = nullptr; /* MFMA form constants */
     uint64_t frag_stride = 0;
-    /* MFMA form: tiles of one material run the MFMA kernel (list a), the
-     * others the VALU z-form kernel of the same state layout (list b) */
+    /* MFMA form: tiles of one material run on the matrix cores, the others
+     * the VALU z-form body in the same kernel; tiles_a lists the mixed
+     * tiles first so that their longer waves start early */
     bool split = false;
     uint32_t* tiles_a = nullptr;
-    uint32_t* tiles_b = nullptr;
-    uint64_t n_tiles_a = 0, n_tiles_b = 0;
-    tile_launch_fn launch_b = nullptr;
+    uint64_t n_tiles_mixed = 0;
     dev_source* src = nullptr;
     uint32_t* tile_src = nullptr;
     dev_record* rec = nullptr;
@@ -679,10 +678,9 @@ int build_tile_tables(mbh_ctx* ctx, bool with_records)
     release(ctx->tile_rec_ptr);
     release(ctx->tile_rec_idx);
     release(ctx->tiles_a);
-    release(ctx->tiles_b);
     ctx->tile_src = ctx->tile_rec_ptr = ctx->tile_rec_idx = nullptr;
-    ctx->tiles_a = ctx->tiles_b = nullptr;
-    ctx->n_tiles_a = ctx->n_tiles_b = 0;
+    ctx->tiles_a = nullptr;
+    ctx->n_tiles_mixed = 0;
     int rc;
     if (ctx->split) {
         /* a tile is uniform when all its 64 cells lie in the grid and have
@@ -700,12 +698,9 @@ int build_tile_tables(mbh_ctx* ctx, bool with_records)
             }
             (uni ? la : lb).push_back((uint32_t)t);
         }
-        ctx->n_tiles_a = la.size();
-        ctx->n_tiles_b = lb.size();
-        if ((rc = upload(ctx, &ctx->tiles_a, la.data(), la.size()))) {
-            return rc;
-        }
-        if ((rc = upload(ctx, &ctx->tiles_b, lb.data(), lb.size()))) {
+        ctx->n_tiles_mixed = lb.size();
+        lb.insert(lb.end(), la.begin(), la.end());
+        if ((rc = upload(ctx, &ctx->tiles_a, lb.data(), lb.size()))) {
             return rc;
         }
     }
@@ -855,25 +850,12 @@ KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
     return k;
 }
 
-/* One temporal block of all tiles: one launch, or with split tile lists
- * the MFMA kernel on the uniform tiles and the VALU kernel on the rest
- * (disjoint owned cells, same input and output buffers). */
+/* One temporal block of all tiles (MFMA form: in the order of the tile
+ * list, tiles with more than one material first). */
 hipError_t launch_block(const mbh_ctx* ctx, KArgs k)
 {
-    if (!ctx->split) {
-        k.tile_list = nullptr;
-        return ctx->launch(k, ctx->num_tiles, ctx->stream);
-    }
-    hipError_t e = hipSuccess;
-    if (ctx->n_tiles_a) {
-        k.tile_list = ctx->tiles_a;
-        e = ctx->launch(k, ctx->n_tiles_a, ctx->stream);
-    }
-    if (e == hipSuccess && ctx->n_tiles_b) {
-        k.tile_list = ctx->tiles_b;
-        e = ctx->launch_b(k, ctx->n_tiles_b, ctx->stream);
-    }
-    return e;
+    k.tile_list = ctx->split ? ctx->tiles_a : nullptr;
+    return ctx->launch(k, ctx->num_tiles, ctx->stream);
 }
 
 int build_graph(mbh_ctx* ctx, int start_buf)
@@ -1262,18 +1244,10 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     HIP_TRY(c, hipMemset(c->n_base, 0, sizeof(uint64_t)));
     HIP_TRY(c, hipDeviceSynchronize());
 
-    /* MFMA form: uniform tiles on the matrix cores, mixed tiles on the
-     * VALU z-form kernel of the same state layout */
+    /* MFMA form: the tile list puts tiles with more than one material
+     * (VALU path inside the kernel) first */
     if (c->form == MBH_FORM_MFMA || c->form == MBH_FORM_MFMA64) {
-        const kernel_entry* kb = find_kernel(
-            c->algo, c->levels,
-            c->fp32 ? MBH_FORM_MIXED : MBH_FORM_ZFORM64, 1);
-        if (!kb) {
-            return fail(nullptr, MBH_ERR_UNSUPPORTED,
-                        "no z-form kernel for the mixed tiles");
-        }
         c->split = true;
-        c->launch_b = kb->fn;
         c->mat_h = mat;
     }
 

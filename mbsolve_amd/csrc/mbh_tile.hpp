@@ -198,13 +198,13 @@ __device__ __forceinline__ double record_value(const KArgs& k, uint32_t type,
 constexpr int TILE_SOURCES = 8;
 constexpr int MAX_BLOCK = 64; /* largest temporal block */
 
+/* One tile t: all steps of the temporal block for the tile's cells. */
 template<class A, int C, typename ST>
-__global__ __launch_bounds__(64) void tile_kernel(KArgs k)
+__device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 {
     __shared__ double s_src[TILE_SOURCES][MAX_BLOCK];
 
     const int lane = threadIdx.x;
-    const uint64_t t = k.tile_list ? k.tile_list[blockIdx.x] : blockIdx.x;
     const int64_t own_lo = (int64_t)(t * k.owned);
     const int64_t own_hi =
         ((own_lo + (int64_t)k.owned) < (int64_t)k.Gown) ? own_lo + k.owned
@@ -402,6 +402,13 @@ __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
             }
         }
     }
+}
+
+template<class A, int C, typename ST>
+__global__ __launch_bounds__(64) void tile_kernel(KArgs k)
+{
+    tile_body<A, C, ST>(k, k.tile_list ? k.tile_list[blockIdx.x]
+                                       : blockIdx.x);
 }
 
 /* ---- launcher registry ------------------------------------------------- */

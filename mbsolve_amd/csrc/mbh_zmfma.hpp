@@ -244,6 +244,18 @@ __global__ __launch_bounds__(64) void zm_kernel(KArgs k)
     const int q = lane >> 4, j = lane & 15;
     const float one_q0 = (q == 0) ? 1.0f : 0.0f;
     const uint64_t t = k.tile_list ? k.tile_list[blockIdx.x] : blockIdx.x;
+    {
+        /* a tile whose 64 cells are not all of one material runs the VALU
+         * z-form body (same state layout); such tiles come first in the
+         * tile list, so they overlap with the MFMA tiles */
+        const int64_t gc = (int64_t)(t * k.owned) - (int64_t)k.halo + lane;
+        const int32_t mc = (gc >= 0 && gc < (int64_t)k.G) ? k.mat[gc] : -1;
+        const int32_t mc0 = __builtin_amdgcn_readfirstlane(mc);
+        if (!__all(mc == mc0)) {
+            tile_body<algo_zform<N>, 1, ST>(k, t);
+            return;
+        }
+    }
     const int64_t own_lo = (int64_t)(t * k.owned);
     const int64_t own_hi =
         ((own_lo + (int64_t)k.owned) < (int64_t)k.Gown) ? own_lo + k.owned
@@ -386,8 +398,6 @@ __global__ __launch_bounds__(64) void zm_kernel(KArgs k)
             }
             p = reduce_scatter4(pp, q) + KC0.b;
         }
-        /* tiles with more than one material (or none) go to the VALU
-         * z-form kernel, which shares this state layout */
 
         /* record copy */
         if ((rec_steps >> s) & 1ull) {
