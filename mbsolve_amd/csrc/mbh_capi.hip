@@ -223,11 +223,10 @@ void zm_build(const double* zq, unsigned char* out)
     for (int t = 0; t < Gm::MT; t++) {
         for (int kc = 0; kc < Gm::KC; kc++) {
             for (int l = 0; l < 64; l++) {
-                const int i = 16 * t + (l % 16), qq = l / 16;
+                const int i = Gm::out_row(t, l % 16), qq = l / 16;
                 for (int sl = 0; sl < 8; sl++) {
                     const int u = 8 * kc + sl;
-                    const int col =
-                        (u < Gm::ROWS) ? Gm::row(u / 4, qq, u % 4) : K;
+                    const int col = (u < Gm::ROWS) ? Gm::slot_row(qq, u) : K;
                     double v = 0.0;
                     if (i < K && col < K) {
                         v = (W[i * K + col] - (i == col ? 1.0 : 0.0)) * sc;
@@ -251,11 +250,13 @@ void zm_build(const double* zq, unsigned char* out)
     for (int l = 0; l < 64; l++) {
         const int qq = l / 16;
         for (int i = 0; i < Gm::PL; i++) {
-            const int plane = Gm::row(i / 2, qq, 2 * (i % 2)) / 2;
-            th[i * 64 + l] = (plane < Gm::B) ? zq[Z::OFF_TH + plane] : 0.0;
+            const int r0 = Gm::slot_row(qq, 2 * i);
+            const int plane = r0 / 2;
+            th[i * 64 + l] =
+                (r0 < K && plane < Gm::B) ? zq[Z::OFF_TH + plane] : 0.0;
         }
         for (int i = 0; i < Gm::ROWS; i++) {
-            const int row = Gm::row(i / 4, qq, i % 4);
+            const int row = Gm::slot_row(qq, i);
             const bool ok = row < K;
             ar[i * 64 + l] = ok ? zq[Z::OFF_A + row] : 0.0;
             cr[i * 64 + l] = ok ? zq[Z::OFF_C + row] : 0.0;

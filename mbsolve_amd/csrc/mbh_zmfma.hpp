@@ -17,13 +17,15 @@
  * Layout.  One wavefront owns a tile of 64 cells.  The FDTD part keeps the
  * tile_kernel layout (lane l = cell l).  The density state of the 64 cells
  * is four 16-cell blocks n = 0..3 in the MFMA accumulator layout: lane
- * (q, j) = (l / 16, l % 16) holds rows 16 t + 4 q + rr (t < MT, rr < 4) of
- * cell 16 n + j.  The MFMA output of M-tile t lands on exactly these rows
- * of the same lane, and the next step's B operand is filled from the
- * lane's own rows: K-slot s of chunk kc of lane group q stands for row
- * 16 (u / 4) + 4 q + u % 4 with u = 8 kc + s, and the host builds the A
- * fragments with the same row order, so no data crosses lanes between
- * steps.  Rotation planes pair rows (2 b, 2 b + 1), which lie in one lane.
+ * (q, j) = (l / 16, l % 16) holds ROWS slots of cell 16 n + j: rows
+ * 16 t + 4 q + rr of the full M-tiles t < FT (rr < 4) and rows
+ * 16 FT + RL q + rr of the last M-tile (rr < RL), where the MFMA output of
+ * each M-tile lands on the same lane (the host orders the A rows so).  The
+ * next step's B operand is filled from the lane's own slots: K-slot s of
+ * chunk kc of lane group q stands for the row of slot u = 8 kc + s, and
+ * the host builds the A columns in the same order, so no data crosses
+ * lanes between steps.  Rotation planes pair rows (2 b, 2 b + 1), which
+ * lie in one lane.
  */
 #ifndef MBH_ZMFMA_HPP
 #define MBH_ZMFMA_HPP
@@ -41,9 +43,16 @@ struct zm_geom {
     static constexpr int K = N * N - 1;
     static constexpr int B = K / 2;
     static constexpr int MT = (K + 15) / 16; /* M-tiles of 16 rows */
-    static constexpr int ROWS = 4 * MT;      /* rows per lane and block */
+    /* The last M-tile holds the V remaining rows; they go to RL (even)
+     * registers per lane group so that no lane carries only padding:
+     * row 16 FT + RL q + rr sits in lane group q, register rr < RL. */
+    static constexpr int FT = MT - 1;
+    static constexpr int V = K - 16 * FT;
+    static constexpr int C4 = (V + 3) / 4;
+    static constexpr int RL = (C4 + (C4 & 1)) < 4 ? (C4 + (C4 & 1)) : 4;
+    static constexpr int ROWS = 4 * FT + RL;  /* state slots per lane */
     static constexpr int KC = (ROWS + 7) / 8; /* K chunks of 32 */
-    static constexpr int PL = 2 * MT;        /* rotation planes per lane */
+    static constexpr int PL = ROWS / 2;       /* rotation planes per lane */
     static constexpr int NP = N - 1;         /* diagonal generators */
     static constexpr int NB = 4;             /* 16-cell blocks per tile */
     /* with a free K slot (u = ROWS) the constant c rides in the product:
@@ -59,24 +68,38 @@ struct zm_geom {
     static constexpr uint64_t OV = PR + (uint64_t)NP * ROWS * 64 * 8; /* o */
     static constexpr uint64_t BYTES = (OV + NP * 8 + 255) & ~uint64_t(255);
 
-    __host__ __device__ static constexpr int row(int t, int q, int rr)
+    /* state row held in slot u of lane group q (>= K: padding) */
+    __host__ __device__ static constexpr int slot_row(int q, int u)
     {
-        return 16 * t + 4 * q + rr;
+        return u < 4 * FT ? 16 * (u / 4) + 4 * q + u % 4
+                          : 16 * FT + RL * q + (u - 4 * FT);
+    }
+    /* state row of output position i (0..15) of M-tile t (>= K: none) */
+    __host__ __device__ static constexpr int out_row(int t, int i)
+    {
+        return t < FT ? 16 * t + i
+                      : ((i % 4) < RL ? 16 * FT + RL * (i / 4) + i % 4 : K);
     }
 };
 
-/* sin and cos of a small angle.  TIER 0: |x| <= 2^-10, Taylor to x^5 /
- * x^4 (relative truncation error below 2e-21); TIER 1: |x| <= 0.25,
- * Taylor to x^13 / x^14; TIER 2: library. */
+/* sin and cos of a small angle x, by wave-uniform tier of max |x|:
+ * TIER 0: |x| <= 2^-13, x - x^3/6 and 1 - x^2/2 (relative truncation
+ * error below 1e-17); TIER 1: |x| <= 2^-10, Taylor to x^5 / x^4 (below
+ * 2e-21 / 3e-19); TIER 2: |x| <= 0.25, Taylor to x^13 / x^14; TIER 3:
+ * library. */
 template<int TIER>
 __device__ __forceinline__ void tier_sincos(double x, double& s, double& c)
 {
     if constexpr (TIER == 0) {
         const double x2 = x * x;
+        s = fma(x * x2, -1.6666666666666666667e-01, x);
+        c = fma(x2, -0.5, 1.0);
+    } else if constexpr (TIER == 1) {
+        const double x2 = x * x;
         s = fma(x * x2, fma(x2, 8.3333333333333333333e-03,
                             -1.6666666666666666667e-01), x);
         c = fma(x2, fma(x2, 4.1666666666666666667e-02, -0.5), 1.0);
-    } else if constexpr (TIER == 1) {
+    } else if constexpr (TIER == 2) {
         taylor_sincos(x, s, c);
     } else {
         sincos(x, &s, &c);
@@ -136,7 +159,7 @@ struct zm_consts {
 };
 
 /* One density step of the tile's four 16-cell blocks.  Phase 1 (VALU,
- * FP64): rotate each block's rows in place (Y := r), this lane's partial
+ * FP64): rotate each block's slots in place (Y := r), this lane's partial
  * a . r into pp, FP16 hi/lo B operands.  Phase 2: the split products into
  * one FP32 accumulator per (block, M-tile), issued chunk by chunk across
  * the twelve independent accumulators.  Phase 3: y' = r + 2^-s acc (+ c
@@ -152,16 +175,13 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
 #pragma unroll
     for (int n = 0; n < Gm::NB; n++) {
 #pragma unroll
-        for (int t = 0; t < Gm::MT; t++) {
-#pragma unroll
-            for (int u = 0; u < 2; u++) {
-                double sn, cs;
-                tier_sincos<TIER>(K_.th[2 * t + u] * eb[n], sn, cs);
-                const double y0 = Y[n][4 * t + 2 * u];
-                const double y1 = Y[n][4 * t + 2 * u + 1];
-                Y[n][4 * t + 2 * u] = fma(cs, y0, -(sn * y1));
-                Y[n][4 * t + 2 * u + 1] = fma(sn, y0, cs * y1);
-            }
+        for (int pl = 0; pl < Gm::PL; pl++) {
+            double sn, cs;
+            tier_sincos<TIER>(K_.th[pl] * eb[n], sn, cs);
+            const double y0 = Y[n][2 * pl];
+            const double y1 = Y[n][2 * pl + 1];
+            Y[n][2 * pl] = fma(cs, y0, -(sn * y1));
+            Y[n][2 * pl + 1] = fma(sn, y0, cs * y1);
         }
         double pa = 0.0;
 #pragma unroll
@@ -223,8 +243,8 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
 #pragma unroll
         for (int t = 0; t < Gm::MT; t++) {
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                const int i = 4 * t + rr;
+            for (int rr = 0; rr < (t < Gm::FT ? 4 : Gm::RL); rr++) {
+                const int i = (t < Gm::FT ? 4 * t : 4 * Gm::FT) + rr;
                 Y[n][i] = Gm::CFOLD
                     ? fma((double)acc[n][t][rr], K_.scale, Y[n][i])
                     : fma((double)acc[n][t][rr], K_.scale,
@@ -291,14 +311,11 @@ __global__ __launch_bounds__(64) void zm_kernel(KArgs k)
         const bool inn = (gn >= 0) && (gn < (int64_t)k.G);
         mb[n] = __shfl(m, 16 * n + j);
 #pragma unroll
-        for (int tt = 0; tt < Gm::MT; tt++) {
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++) {
-                const int row = Gm::row(tt, q, rr);
-                Y[n][4 * tt + rr] = (inn && row < Gm::K)
-                    ? ld<ST>(k.d_in, (uint64_t)row * k.dstride + gn)
-                    : 0.0;
-            }
+        for (int u = 0; u < Gm::ROWS; u++) {
+            const int row = Gm::slot_row(q, u);
+            Y[n][u] = (inn && row < Gm::K)
+                ? ld<ST>(k.d_in, (uint64_t)row * k.dstride + gn)
+                : 0.0;
         }
     }
 
@@ -386,15 +403,18 @@ __global__ __launch_bounds__(64) void zm_kernel(KArgs k)
                 emax = fmax(emax, fabs(eb[nb]));
             }
             const double xmax = KC0.thmax * emax;
-            const int tier = __all(xmax <= 0x1p-10) ? 0
-                : (__all(xmax <= TAYLOR_MAX) ? 1 : 2);
+            const int tier = __all(xmax <= 0x1p-13) ? 0
+                : (__all(xmax <= 0x1p-10) ? 1
+                   : (__all(xmax <= TAYLOR_MAX) ? 2 : 3));
             double pp[Gm::NB];
             if (tier == 0) {
                 zm_step<N, 0>(KC0, eb, one_q0, Y, pp);
             } else if (tier == 1) {
                 zm_step<N, 1>(KC0, eb, one_q0, Y, pp);
-            } else {
+            } else if (tier == 2) {
                 zm_step<N, 2>(KC0, eb, one_q0, Y, pp);
+            } else {
+                zm_step<N, 3>(KC0, eb, one_q0, Y, pp);
             }
             p = reduce_scatter4(pp, q) + KC0.b;
         }
@@ -492,14 +512,10 @@ __global__ __launch_bounds__(64) void zm_kernel(KArgs k)
         const int64_t gn = g0 + 16 * n + j;
         if (gn >= own_lo && gn < own_hi) {
 #pragma unroll
-            for (int tt = 0; tt < Gm::MT; tt++) {
-#pragma unroll
-                for (int rr = 0; rr < 4; rr++) {
-                    const int row = Gm::row(tt, q, rr);
-                    if (row < Gm::K) {
-                        st<ST>(k.d_out, (uint64_t)row * k.dstride + gn,
-                               Y[n][4 * tt + rr]);
-                    }
+            for (int u = 0; u < Gm::ROWS; u++) {
+                const int row = Gm::slot_row(q, u);
+                if (row < Gm::K) {
+                    st<ST>(k.d_out, (uint64_t)row * k.dstride + gn, Y[n][u]);
                 }
             }
         }
