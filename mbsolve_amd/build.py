@@ -89,7 +89,35 @@ def build(verbose=True, jobs=8):
           "-L" + PKG, "-lmbsolve", "-lmbsolve_hip", "-Wl,-rpath,$ORIGIN",
           "-o", mod])
     log("built", mod)
-    return [hip_lib, lib, mod]
+    out = [hip_lib, lib, mod]
+    out += _build_writer_hdf5(ext, log)
+    return out
+
+
+def _build_writer_hdf5(ext, log):
+    """Optional HDF5 writer (needs the HDF5 C library and headers)."""
+    root = os.environ.get("MBSOLVE_HDF5_ROOT", "/opt/conda")
+    inc, libdir = os.path.join(root, "include"), os.path.join(root, "lib")
+    if not (os.path.exists(os.path.join(inc, "hdf5.h")) and
+            os.path.exists(os.path.join(libdir, "libhdf5.so"))):
+        log("HDF5 not found under %s: writer-hdf5 not built" % root)
+        return []
+    obj = os.path.join(BUILD, "writer_hdf5.o")
+    _run([CXX] + CXX_FLAGS + ["-isystem", inc, "-c",
+                              os.path.join(PKG, "cpp", "src",
+                                           "writer_hdf5.cpp"), "-o", obj])
+    wlib = os.path.join(PKG, "libmbsolve_writer_hdf5.so")
+    _run([CXX, "-shared", "-fPIC", obj, "-L" + PKG, "-lmbsolve",
+          "-L" + libdir, "-lhdf5", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath," + libdir, "-o", wlib])
+    log("built", wlib)
+    mod = os.path.join(PKG, "_writerhdf5" + ext)
+    _run([CXX] + CXX_FLAGS + _pybind_includes() +
+         ["-shared", os.path.join(PKG, "cpp", "python", "writerhdf5.cpp"),
+          "-L" + PKG, "-lmbsolve_writer_hdf5", "-lmbsolve",
+          "-Wl,-rpath,$ORIGIN", "-o", mod])
+    log("built", mod)
+    return [wlib, mod]
 
 
 if __name__ == "__main__":

@@ -352,6 +352,31 @@ PYBIND11_MODULE(_mbsolve, m)
         .def("set_ic_magnetic", &scenario::set_ic_magnetic);
 
     py::class_<result, std::shared_ptr<result> >(m, "result")
+        .def(py::init<const std::string&, unsigned int, unsigned int, bool>(),
+             py::arg("name"), py::arg("cols"), py::arg("rows"),
+             py::arg("is_complex") = false)
+        .def("set_data",
+             [](result& r, py::array_t<double, py::array::c_style |
+                                                   py::array::forcecast> re,
+                py::object im) {
+                 /* fills the real (and imaginary) part, row-major
+                  * [rows][cols]: for results built outside a solver */
+                 if ((size_t)re.size() != (size_t)r.get_count()) {
+                     throw std::invalid_argument("result size mismatch");
+                 }
+                 std::copy(re.data(), re.data() + re.size(),
+                           r.get_data_real_it(0, 0));
+                 if (!im.is_none()) {
+                     auto ia = im.cast<py::array_t<
+                         double, py::array::c_style | py::array::forcecast> >();
+                     if ((size_t)ia.size() != (size_t)r.get_count()) {
+                         throw std::invalid_argument("result size mismatch");
+                     }
+                     std::copy(ia.data(), ia.data() + ia.size(),
+                               r.get_data_imag_it(0, 0));
+                 }
+             },
+             py::arg("real"), py::arg("imag") = py::none())
         .def("get_name", &result::get_name)
         .def("get_cols", &result::get_cols)
         .def("get_rows", &result::get_rows)

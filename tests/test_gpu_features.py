@@ -185,3 +185,22 @@ def test_ziolkowski_sit_reference_curve(gpu):
     x_peak = x[win][np.argmax(inv[75][win])]
     assert abs(x_peak - x_peak_ref) < 1.0e-6
     assert inv[75][win].max() > 0.95
+
+
+def test_hdf5_writer_after_run(gpu, tmp_path):
+    """Results of a GPU run written by the hdf5 writer read back unchanged
+    (writer-hdf5/src/writer_hdf5.cpp layout)."""
+    hdf5 = pytest.importorskip("mbsolve_amd.writerhdf5")
+    spec = setups.sit_2lvl(num_gridpoints=500, endtime=0.2e-12,
+                           num_records=9)
+    dev, scen = setups.build(spec)
+    sol = mb.create_hip_solver("hip-fdtd-2lvl-reg-cayley", dev, scen)
+    sol.run()
+    fn = str(tmp_path / "sit.hdf")
+    mb.writer.create_instance("hdf5").write(fn, sol.get_results(), dev,
+                                            scen)
+    f = hdf5.read_file(fn)
+    assert f["attributes"]["timestep_size"] == scen.get_timestep_size()
+    for r in sol.get_results():
+        assert np.array_equal(f["results"][r.get_name()]["real"],
+                              r.get_data_real())
