@@ -235,9 +235,17 @@ void zm_build(const double* zq, unsigned char* out)
                         v = cv[i] * sc;
                     }
                     const _Float16 hi = (_Float16)v;
-                    const _Float16 lo = (_Float16)(v - (double)hi);
-                    fr[(((t * Gm::KC + kc) * 2 + 0) * 64 + l) * 8 + sl] = hi;
-                    fr[(((t * Gm::KC + kc) * 2 + 1) * 64 + l) * 8 + sl] = lo;
+                    const double r1 = v - (double)hi;
+                    const _Float16 mid = (_Float16)r1;
+                    const _Float16 lo = (_Float16)(r1 - (double)mid);
+                    const int f = (t * Gm::KC + kc) * Gm::AP;
+                    fr[(f * 64 + l) * 8 + sl] = hi;
+                    if (Gm::AP == 3) {
+                        fr[((f + 1) * 64 + l) * 8 + sl] = mid;
+                        fr[((f + 2) * 64 + l) * 8 + sl] = lo;
+                    } else {
+                        fr[((f + 1) * 64 + l) * 8 + sl] = (_Float16)r1;
+                    }
                 }
             }
         }

@@ -7,12 +7,13 @@
  *   r = R(e) y,  p = a . r + b,  y' = r + c + (W - I) r.
  * The affine part and the rotation stay in FP64; the product (W - I) r,
  * a K x K matrix shared by all cells of a material times the K x 64 block
- * of the tile's rotated states, runs as v_mfma_f32_16x16x32_f16 with both
- * operands split into FP16 hi + lo parts (A_hi r_hi + A_hi r_lo +
- * A_lo r_hi, FP32 accumulation).  (W - I) is scaled by a power of two so
- * that its entries use the FP16 range; the products are scaled back in
- * the FP64 accumulation.  W - I is O(rate * dt), so the split's relative
- * error of about 2^-21 enters the state at about 1e-10 per step.
+ * of the tile's rotated states, runs as v_mfma_f32_16x16x32_f16 on FP16
+ * pieces: A = W - I as hi + lo (or hi + mid + lo with ZM_A_PIECES = 3),
+ * r as hi + lo; products A_hi r_hi + A_hi r_lo (+ A_mid r_hi) + A_lo r_hi
+ * with FP32 accumulation.  A is scaled by a power of two into the FP16
+ * range; the products are scaled back in the FP64 accumulation.  The
+ * relative error of the product is about 2^-22 per step.  The A fragments
+ * live in LDS (each lane reads back only its own slots).
  *
  * Layout.  One wavefront owns a tile of 64 cells.  The FDTD part keeps the
  * tile_kernel layout (lane l = cell l).  The density state of the 64 cells
@@ -33,6 +34,10 @@
 #include "mbh_tile.hpp"
 
 namespace mbh {
+
+#ifndef ZM_A_PIECES
+#define ZM_A_PIECES 2
+#endif
 
 typedef _Float16 zm_h8 __attribute__((ext_vector_type(8)));
 typedef float zm_f4 __attribute__((ext_vector_type(4)));
@@ -60,8 +65,11 @@ struct zm_geom {
     static constexpr bool CFOLD = (ROWS % 8) != 0;
     /* header: [0] 2^-s (scale of the product) [1] max theta_dt
      *         [2] b (p offset) [3] has_qm */
-    static constexpr uint64_t FRAG = 32;     /* zm_h8 [MT][KC][2][64] */
-    static constexpr uint64_t TH = FRAG + (uint64_t)MT * KC * 2 * 64 * 16;
+    /* FP16 pieces of A = W - I: hi + lo (AP = 2) or hi + mid + lo (3) */
+    static constexpr int AP = ZM_A_PIECES;
+    static constexpr int NF = MT * KC * AP;  /* A fragments */
+    static constexpr uint64_t FRAG = 32;     /* zm_h8 [MT][KC][AP][64] */
+    static constexpr uint64_t TH = FRAG + (uint64_t)NF * 64 * 16;
     static constexpr uint64_t AR = TH + (uint64_t)PL * 64 * 8;   /* a rows */
     static constexpr uint64_t CR = AR + (uint64_t)ROWS * 64 * 8; /* c rows */
     static constexpr uint64_t PR = CR + (uint64_t)ROWS * 64 * 8; /* P rows */
@@ -124,20 +132,17 @@ __device__ __forceinline__ double reduce_scatter4(const double* v, int q)
 template<int N>
 struct zm_consts {
     using Gm = zm_geom<N>;
-    zm_h8 ahi[Gm::MT][Gm::KC], alo[Gm::MT][Gm::KC];
     double th[Gm::PL], ar[Gm::ROWS], cr[Gm::ROWS];
     double scale, thmax, b;
 
-    __device__ void load(const char* base, int lane)
+    /* registers: theta, a, c and the header; the A fragments go to this
+     * lane's slots of the LDS array sf ([NF][64]) */
+    __device__ void load(const char* base, int lane, zm_h8 (*sf)[64])
     {
         const zm_h8* f = reinterpret_cast<const zm_h8*>(base + Gm::FRAG);
 #pragma unroll
-        for (int t = 0; t < Gm::MT; t++) {
-#pragma unroll
-            for (int kc = 0; kc < Gm::KC; kc++) {
-                ahi[t][kc] = f[((t * Gm::KC + kc) * 2 + 0) * 64 + lane];
-                alo[t][kc] = f[((t * Gm::KC + kc) * 2 + 1) * 64 + lane];
-            }
+        for (int i = 0; i < Gm::NF; i++) {
+            sf[i][lane] = f[i * 64 + lane];
         }
         const double* th_ = reinterpret_cast<const double*>(base + Gm::TH);
         const double* ar_ = reinterpret_cast<const double*>(base + Gm::AR);
@@ -166,6 +171,7 @@ struct zm_consts {
  * when c is not carried in the product). */
 template<int N, int TIER>
 __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
+                                        const zm_h8 (*sf)[64], int lane,
                                         const double* eb, float one_q0,
                                         double (*Y)[zm_geom<N>::ROWS],
                                         double* pp)
@@ -211,30 +217,38 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
             acc[n][t] = zm_f4{ 0.0f, 0.0f, 0.0f, 0.0f };
         }
     }
+    /* (W - I) r = A_hi r_hi + A_hi r_lo [+ A_mid r_hi] + A_lo r_hi: W - I
+     * carried to about 2^-22 (two pieces) or 2^-33 (three), r to about
+     * 2^-22 (a rounding that changes from step to step) */
 #pragma unroll
-    for (int kc = 0; kc < Gm::KC; kc++) {
+    for (int t = 0; t < Gm::MT; t++) {
 #pragma unroll
-        for (int n = 0; n < Gm::NB; n++) {
+        for (int kc = 0; kc < Gm::KC; kc++) {
+            const int f = (t * Gm::KC + kc) * Gm::AP;
+            const zm_h8 ahi = sf[f][lane];
+            const zm_h8 alo = sf[f + Gm::AP - 1][lane];
 #pragma unroll
-            for (int t = 0; t < Gm::MT; t++) {
+            for (int n = 0; n < Gm::NB; n++) {
                 acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                    K_.ahi[t][kc], bh[n][kc], acc[n][t], 0, 0, 0);
+                    ahi, bh[n][kc], acc[n][t], 0, 0, 0);
             }
-        }
 #pragma unroll
-        for (int n = 0; n < Gm::NB; n++) {
-#pragma unroll
-            for (int t = 0; t < Gm::MT; t++) {
+            for (int n = 0; n < Gm::NB; n++) {
                 acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                    K_.ahi[t][kc], bl[n][kc], acc[n][t], 0, 0, 0);
+                    ahi, bl[n][kc], acc[n][t], 0, 0, 0);
             }
-        }
+            if constexpr (Gm::AP == 3) {
+                const zm_h8 amid = sf[f + 1][lane];
 #pragma unroll
-        for (int n = 0; n < Gm::NB; n++) {
+                for (int n = 0; n < Gm::NB; n++) {
+                    acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        amid, bh[n][kc], acc[n][t], 0, 0, 0);
+                }
+            }
 #pragma unroll
-            for (int t = 0; t < Gm::MT; t++) {
+            for (int n = 0; n < Gm::NB; n++) {
                 acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                    K_.alo[t][kc], bh[n][kc], acc[n][t], 0, 0, 0);
+                    alo, bh[n][kc], acc[n][t], 0, 0, 0);
             }
         }
     }
@@ -327,9 +341,10 @@ __global__ __launch_bounds__(64) void zm_kernel(KArgs k)
     const bool q0_active = uniform && m0 >= 0 &&
         as_const(reinterpret_cast<const double*>(
             fbase + (uint64_t)m0 * k.frag_stride))[3] != 0.0;
+    __shared__ zm_h8 s_frag[Gm::NF][64];
     zm_consts<N> KC0;
     if (q0_active) {
-        KC0.load(fbase + (uint64_t)m0 * k.frag_stride, lane);
+        KC0.load(fbase + (uint64_t)m0 * k.frag_stride, lane, s_frag);
     }
     bool my_qm = false; /* this lane's cell has a density */
     if (m >= 0) {
@@ -408,13 +423,13 @@ __global__ __launch_bounds__(64) void zm_kernel(KArgs k)
                    : (__all(xmax <= TAYLOR_MAX) ? 2 : 3));
             double pp[Gm::NB];
             if (tier == 0) {
-                zm_step<N, 0>(KC0, eb, one_q0, Y, pp);
+                zm_step<N, 0>(KC0, s_frag, lane, eb, one_q0, Y, pp);
             } else if (tier == 1) {
-                zm_step<N, 1>(KC0, eb, one_q0, Y, pp);
+                zm_step<N, 1>(KC0, s_frag, lane, eb, one_q0, Y, pp);
             } else if (tier == 2) {
-                zm_step<N, 2>(KC0, eb, one_q0, Y, pp);
+                zm_step<N, 2>(KC0, s_frag, lane, eb, one_q0, Y, pp);
             } else {
-                zm_step<N, 3>(KC0, eb, one_q0, Y, pp);
+                zm_step<N, 3>(KC0, s_frag, lane, eb, one_q0, Y, pp);
             }
             p = reduce_scatter4(pp, q) + KC0.b;
         }
