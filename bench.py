@@ -58,7 +58,7 @@ DEFAULTS = {"v3": ("mixed,zform", "", 0),
 MFMA_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA
 # v_mfma_f32_16x16x32_f16 per 16-cell block and step: 3 split products x
 # M-tiles x K chunks (six levels: 3 x 3 x 2)
-MFMA_PER_BLOCK = {6: 18, 5: 6, 4: 3}
+MFMA_PER_BLOCK = {6: 18, 5: 6, 4: 3, 3: 4}
 MFMA_FLOPS = 2 * 16 * 16 * 32
 
 
@@ -358,7 +358,7 @@ def main():
     if head.startswith("mfma"):
         # matrix-core work of the uniform tiles (the few tiles that straddle
         # runs or the grid ends take the VALU kernel)
-        lev = 6 if args.config in ("qcl6", "sweep6") else 0
+        lev = {"qcl6": 6, "sweep6": 6, "v3": 3}.get(args.config, 0)
         blocks = 4 * info["num_tiles"] * launches * info["temporal_block"]
         hw = blocks * MFMA_PER_BLOCK.get(lev, 0) * MFMA_FLOPS
         t = hw / (r["dev_ms"] * 1e-3) / 1e12

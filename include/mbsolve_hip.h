@@ -82,7 +82,7 @@ enum {
      * Bloch-form two-level Cayley) without the FP32 rounding of the stored
      * state. */
     MBH_FORM_ZFORM64 = 2,
-    /* Rodrigues z-form, N = 4..6, FP32 state storage: rotation, affine part
+    /* Rodrigues z-form, N = 3..6, FP32 state storage: rotation, affine part
      * and p in FP64, the generator product (W - I) r on the matrix cores as
      * FP16 hi + lo split products with FP32 accumulation. */
     MBH_FORM_MFMA = 3,

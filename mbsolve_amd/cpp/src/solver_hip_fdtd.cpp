@@ -93,7 +93,7 @@ parse_hip_solver_name(const std::string& name_in, density_algo& algo,
     num_lvl = lv[0] - '0';
     if (alg == "rodr") {
         algo = density_algo::cvr_rodr;
-        return !mfma || num_lvl >= 4;
+        return !mfma || num_lvl >= 3;
     }
     if (alg == "reg-cayley" && (num_lvl == 2 || num_lvl == 3)) {
         algo = density_algo::reg_cayley;
@@ -278,7 +278,7 @@ solver_hip_loader::solver_hip_loader()
         for (unsigned int n = 2; n <= 6; n++) {
             for (const char* f : { "", "-mixed", "-zform", "-mfma",
                                    "-mfma64" }) {
-                if (n < 4 && std::string(f).rfind("-mfma", 0) == 0) {
+                if (n < 3 && std::string(f).rfind("-mfma", 0) == 0) {
                     continue;
                 }
                 names.push_back("hip-fdtd-" + std::to_string(n) + "lvl-rodr" +

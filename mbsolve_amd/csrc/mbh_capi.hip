@@ -283,6 +283,7 @@ void zm_build(const double* zq, unsigned char* out)
 uint64_t zm_bytes(uint32_t levels)
 {
     switch (levels) {
+    case 3: return zm_geom<3>::BYTES;
     case 4: return zm_geom<4>::BYTES;
     case 5: return zm_geom<5>::BYTES;
     case 6: return zm_geom<6>::BYTES;
@@ -293,6 +294,7 @@ uint64_t zm_bytes(uint32_t levels)
 void zm_build_levels(uint32_t levels, const double* zq, unsigned char* out)
 {
     switch (levels) {
+    case 3: zm_build<3>(zq, out); break;
     case 4: zm_build<4>(zq, out); break;
     case 5: zm_build<5>(zq, out); break;
     case 6: zm_build<6>(zq, out); break;
@@ -521,9 +523,9 @@ int validate(const mbh_problem* p, uint32_t r, const mbh_problem* p0,
         }
     }
     const bool mfma = (form == MBH_FORM_MFMA || form == MBH_FORM_MFMA64);
-    if (mfma && (p->algo != MBH_ALGO_CVR_RODR || p->num_levels < 4)) {
+    if (mfma && (p->algo != MBH_ALGO_CVR_RODR || p->num_levels < 3)) {
         return fail(nullptr, MBH_ERR_UNSUPPORTED,
-                    "run %u: the MFMA form is for Rodrigues with 4 to 6 "
+                    "run %u: the MFMA form is for Rodrigues with 3 to 6 "
                     "levels",
                     r);
     }

@@ -1,15 +1,17 @@
 /*
  * Kernel instantiations: Rodrigues z-form with the generator product on
- * MFMA, N = 4..6, FP32 state storage (MBH_FORM_MFMA) and FP64 state
+ * MFMA, N = 3..6, FP32 state storage (MBH_FORM_MFMA) and FP64 state
  * storage (MBH_FORM_MFMA64).
  */
 #include "mbh_zmfma.hpp"
 
 namespace mbh {
 
+template __global__ void zm_kernel<3, float>(KArgs);
 template __global__ void zm_kernel<4, float>(KArgs);
 template __global__ void zm_kernel<5, float>(KArgs);
 template __global__ void zm_kernel<6, float>(KArgs);
+template __global__ void zm_kernel<3, double>(KArgs);
 template __global__ void zm_kernel<4, double>(KArgs);
 template __global__ void zm_kernel<5, double>(KArgs);
 template __global__ void zm_kernel<6, double>(KArgs);
@@ -19,9 +21,11 @@ template __global__ void zm_kernel<6, double>(KArgs);
     kernel_entry { MBH_ALGO_CVR_RODR, LEV, FORM, 1, &launch_zm<LEV, ST> }
 
 const kernel_entry k_table_zm[] = {
+    MBH_ZMENTRY(3, MBH_FORM_MFMA, float),
     MBH_ZMENTRY(4, MBH_FORM_MFMA, float),
     MBH_ZMENTRY(5, MBH_FORM_MFMA, float),
     MBH_ZMENTRY(6, MBH_FORM_MFMA, float),
+    MBH_ZMENTRY(3, MBH_FORM_MFMA64, double),
     MBH_ZMENTRY(4, MBH_FORM_MFMA64, double),
     MBH_ZMENTRY(5, MBH_FORM_MFMA64, double),
     MBH_ZMENTRY(6, MBH_FORM_MFMA64, double),

@@ -1,6 +1,6 @@
 /*
  * Rodrigues z-form tile kernel with the generator product on the matrix
- * cores (gfx950 MFMA), for N = 4..6 levels (K = N^2 - 1 = 15, 24, 35).
+ * cores (gfx950 MFMA), for N = 3..6 levels (K = N^2 - 1 = 8, 15, 24, 35).
  *
  * One step of the z-form (lindblad_cvr_rodr::update,
  * algo_lindblad_cvr_rodr.hpp:234-268, restated in the rotated basis):
@@ -65,8 +65,11 @@ struct zm_geom {
     static constexpr bool CFOLD = (ROWS % 8) != 0;
     /* header: [0] 2^-s (scale of the product) [1] max theta_dt
      *         [2] b (p offset) [3] has_qm */
-    /* FP16 pieces of A = W - I: hi + lo (AP = 2) or hi + mid + lo (3) */
-    static constexpr int AP = ZM_A_PIECES;
+    /* FP16 pieces of A = W - I: hi + lo (AP = 2) or hi + mid + lo (3).
+     * Three levels at optical frequencies put the free precession in W - I
+     * (omega dt ~ 5e-3 per step): its representation error would act as a
+     * frequency shift, so it gets three pieces. */
+    static constexpr int AP = (N <= 3) ? 3 : ZM_A_PIECES;
     static constexpr int NF = MT * KC * AP;  /* A fragments */
     static constexpr uint64_t FRAG = 32;     /* zm_h8 [MT][KC][AP][64] */
     static constexpr uint64_t TH = FRAG + (uint64_t)NF * 64 * 16;

@@ -143,12 +143,19 @@ def _qcl6_with_populations():
 
 
 @pytest.mark.parametrize("form", ["mfma", "mfma64"])
-@pytest.mark.parametrize("make", [
-    _qcl6_with_populations,
-    lambda: setups.marskar2011(4, num_gridpoints=700, endtime=0.2e-12),
-    lambda: setups.marskar2011(5, num_gridpoints=400, endtime=0.1e-12),
-], ids=["qcl6", "marskar4", "marskar5"])
-def test_mfma_forms_vs_oracle(gpu, form, make):
+@pytest.mark.parametrize("make,tol", [
+    (_qcl6_with_populations, MFMA_TOL),
+    (lambda: setups.marskar2011(4, num_gridpoints=700, endtime=0.2e-12),
+     MFMA_TOL),
+    (lambda: setups.marskar2011(5, num_gridpoints=400, endtime=0.1e-12),
+     MFMA_TOL),
+    # three levels at optical frequencies, 7371 steps: W - I carries the
+    # free precession (~5e-3 per step), so the split product's per-step
+    # rounding accumulates faster
+    (lambda: setups.v3_rodr(num_gridpoints=2048, endtime=0.9e-12,
+                            num_records=32), 5e-6),
+], ids=["qcl6", "marskar4", "marskar5", "v3"])
+def test_mfma_forms_vs_oracle(gpu, form, make, tol):
     """Rodrigues z-form with the generator product on MFMA (split FP16,
     FP32 accumulation) against the oracle: fields relative to their
     maximum, populations on the trace-1 scale."""
@@ -160,7 +167,7 @@ def test_mfma_forms_vs_oracle(gpu, form, make):
             err = float(np.max(np.abs(a - b)))
         else:
             err = rel_err(a, b)
-        assert err <= MFMA_TOL, (r.get_name(), err)
+        assert err <= tol, (r.get_name(), err)
 
 
 def test_mfma_batch_mixed_tiles(gpu):
