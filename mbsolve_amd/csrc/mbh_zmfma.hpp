@@ -180,7 +180,12 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
                                         double* pp)
 {
     using Gm = zm_geom<N>;
-    zm_h8 bh[Gm::NB][Gm::KC], bl[Gm::NB][Gm::KC];
+    zm_h8 af[Gm::NF];
+#pragma unroll
+    for (int f = 0; f < Gm::NF; f++) {
+        af[f] = sf[f][lane];
+    }
+    zm_f4 acc[Gm::NB][Gm::MT];
 #pragma unroll
     for (int n = 0; n < Gm::NB; n++) {
 #pragma unroll
@@ -198,6 +203,7 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
             pa = fma(K_.ar[i], Y[n][i], pa);
         }
         pp[n] = pa;
+        zm_h8 bh[Gm::KC], bl[Gm::KC];
 #pragma unroll
         for (int kc = 0; kc < Gm::KC; kc++) {
 #pragma unroll
@@ -207,52 +213,32 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
                     ? (float)Y[n][u]
                     : ((Gm::CFOLD && u == Gm::ROWS) ? one_q0 : 0.0f);
                 const _Float16 hi = (_Float16)v;
-                bh[n][kc][sl] = hi;
-                bl[n][kc][sl] = (_Float16)(v - (float)hi);
+                bh[kc][sl] = hi;
+                bl[kc][sl] = (_Float16)(v - (float)hi);
             }
         }
-    }
-    zm_f4 acc[Gm::NB][Gm::MT];
-#pragma unroll
-    for (int n = 0; n < Gm::NB; n++) {
+        /* (W - I) r of this block = A_hi r_hi + A_hi r_lo [+ A_mid r_hi]
+         * + A_lo r_hi: W - I to about 2^-22 (two pieces) or 2^-33 (three),
+         * r to about 2^-22 (a rounding that changes from step to step).
+         * Issued here, so the next block's VALU work overlaps them. */
 #pragma unroll
         for (int t = 0; t < Gm::MT; t++) {
-            acc[n][t] = zm_f4{ 0.0f, 0.0f, 0.0f, 0.0f };
-        }
-    }
-    /* (W - I) r = A_hi r_hi + A_hi r_lo [+ A_mid r_hi] + A_lo r_hi: W - I
-     * carried to about 2^-22 (two pieces) or 2^-33 (three), r to about
-     * 2^-22 (a rounding that changes from step to step) */
+            zm_f4 a = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
-    for (int t = 0; t < Gm::MT; t++) {
-#pragma unroll
-        for (int kc = 0; kc < Gm::KC; kc++) {
-            const int f = (t * Gm::KC + kc) * Gm::AP;
-            const zm_h8 ahi = sf[f][lane];
-            const zm_h8 alo = sf[f + Gm::AP - 1][lane];
-#pragma unroll
-            for (int n = 0; n < Gm::NB; n++) {
-                acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                    ahi, bh[n][kc], acc[n][t], 0, 0, 0);
-            }
-#pragma unroll
-            for (int n = 0; n < Gm::NB; n++) {
-                acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                    ahi, bl[n][kc], acc[n][t], 0, 0, 0);
-            }
-            if constexpr (Gm::AP == 3) {
-                const zm_h8 amid = sf[f + 1][lane];
-#pragma unroll
-                for (int n = 0; n < Gm::NB; n++) {
-                    acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                        amid, bh[n][kc], acc[n][t], 0, 0, 0);
+            for (int kc = 0; kc < Gm::KC; kc++) {
+                const int f = (t * Gm::KC + kc) * Gm::AP;
+                a = __builtin_amdgcn_mfma_f32_16x16x32_f16(af[f], bh[kc], a,
+                                                           0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_16x16x32_f16(af[f], bl[kc], a,
+                                                           0, 0, 0);
+                if constexpr (Gm::AP == 3) {
+                    a = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        af[f + 1], bh[kc], a, 0, 0, 0);
                 }
+                a = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                    af[f + Gm::AP - 1], bh[kc], a, 0, 0, 0);
             }
-#pragma unroll
-            for (int n = 0; n < Gm::NB; n++) {
-                acc[n][t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                    alo, bh[n][kc], acc[n][t], 0, 0, 0);
-            }
+            acc[n][t] = a;
         }
     }
 #pragma unroll
