@@ -8,7 +8,8 @@ copy.  Other workloads: "sit2" (two-level self-induced transparency,
 regularized Cayley, 8192 cells, 1.5 ps), "qcl6" (six-level gain medium,
 Rodrigues, 2^20 cells, the first 20000 of 4e6 steps), "sweep2" (512
 two-level runs of 2^12..2^17 cells, 49114 shared steps), "sweep6" (64
-six-level runs of 2^14..2^18 cells, the first 20000 of 4e6 steps).
+six-level runs of 2^14..2^18 cells, the first 20000 of 4e6 steps),
+"maxwell" (the pure Maxwell stencil on 2^27 cells, 4000 steps).
 
 Candidate fast forms are timed on the same steps and their records compared
 with an FP64 reference run (relative error max|x - x_ref| / max|x_ref|, per
@@ -22,7 +23,7 @@ synchronisation.  A bounded sample of the CPU FP64 oracle (the restatement
 of the reference's cpu-fdtd solver) is timed on the host cores.
 
     python bench.py [--steps K] [--warmup W]
-                    [--config v3|sit2|qcl6|sweep2|sweep6]
+                    [--config v3|sit2|qcl6|sweep2|sweep6|maxwell]
                     [--forms a,b] [--no-cpu] [--no-fp64]
 """
 import argparse
@@ -45,16 +46,18 @@ ALGO_BYTES = {("v3", "f32"): 80, ("v3", "f64"): 160,
               ("sit2", "f32"): 48, ("sit2", "f64"): 96,
               ("qcl6", "f32"): 296, ("qcl6", "f64"): 592,
               ("sweep2", "f32"): 48, ("sweep2", "f64"): 96,
-              ("sweep6", "f32"): 296, ("sweep6", "f64"): 592}
+              ("sweep6", "f32"): 296, ("sweep6", "f64"): 592,
+              ("maxwell", "f32"): 16, ("maxwell", "f64"): 32}
 ALGO_FLOPS = {"v3": 168, "sit2": 120, "qcl6": 2622, "sweep2": 120,
-              "sweep6": 2622}
+              "sweep6": 2622, "maxwell": 10}
 # per workload: default candidate forms, reference form for the errors,
 # default steps of the plain FP64 timing (0: same as the candidates)
 DEFAULTS = {"v3": ("mixed,zform", "", 0),
             "sit2": ("mixed,zform", "", 0),
             "sweep2": ("mixed,zform", "", 0),
             "qcl6": ("mfma,mfma64,zform", "zform", 400),
-            "sweep6": ("mfma,mfma64,zform", "zform", 400)}
+            "sweep6": ("mfma,mfma64,zform", "zform", 400),
+            "maxwell": ("mixed", "", 0)}
 MFMA_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA
 # v_mfma_f32_16x16x32_f16 per 16-cell block and step: 3 split products x
 # M-tiles x K chunks (six levels: 3 x 3 x 2)
@@ -80,6 +83,9 @@ def make_specs(config):
     if config == "sweep2":
         return setups.sit_2lvl_sweep(), "512 two-level SIT runs, " \
             "2^12..2^17 cells, regularized Cayley, 49114 steps"
+    if config == "maxwell":
+        return [setups.maxwell_line()], "pure Maxwell FDTD (no density), " \
+            "2^27 cells, 4000 steps"
     if config == "sweep6":
         return setups.qcl6_sweep(num_timesteps=20000), "64 six-level " \
             "gain-medium runs, 2^14..2^18 cells, Rodrigues, first 20000 " \
@@ -139,9 +145,23 @@ def timed_run(sol, warmup, steps, chunk_s=20.0):
     return dev_ms, wall
 
 
-def cpu_baseline(specs, form_name, seconds=15.0):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(specs, form_name, seconds=15.0, reps=3):
     """Bounded sample of the CPU FP64 oracle on the same workload: the full
-    grid of run 0 for as many steps as fit in about `seconds`."""
+    grid of run 0, `reps` timed pieces of as many steps as fit in about
+    `seconds` together; the median rate is reported.  OpenMP threads are
+    pinned to cores (OMP_PROC_BIND=close, OMP_PLACES=cores, set in main
+    before the oracle library loads)."""
     import mbsolve_amd.lib as mb
     from mbsolve_amd import setups
     import oracle
@@ -152,20 +172,32 @@ def cpu_baseline(specs, form_name, seconds=15.0):
         min(16, os.cpu_count() or 1)
     run = oracle.OracleRun(prob)
     nx = int(prob["num_gridpoints"])
-    probe = 20
+    probe = 10
+    run.run(0, probe, threads=threads)  # warm-up (thread start, caches)
     t0 = time.perf_counter()
-    run.run(0, probe, threads=threads)
+    run.run(probe, 2 * probe, threads=threads)
     dt = time.perf_counter() - t0
-    nsteps = int(max(probe, min(prob["num_timesteps"] - probe,
-                                seconds / max(dt, 1e-9) * probe)))
-    t0 = time.perf_counter()
-    run.run(probe, probe + nsteps, threads=threads)
-    el = time.perf_counter() - t0
-    return {"value": nx * nsteps / el, "unit": "cell-updates/s",
+    avail = int(prob["num_timesteps"]) - 2 * probe
+    per = int(max(1, min(avail // reps,
+                         seconds / reps / max(dt, 1e-9) * probe)))
+    rates, total = [], 0.0
+    n = 2 * probe
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run.run(n, n + per, threads=threads)
+        el = time.perf_counter() - t0
+        rates.append(nx * per / el)
+        total += el
+        n += per
+    rates.sort()
+    return {"value": rates[len(rates) // 2], "unit": "cell-updates/s",
             "cores": threads, "kind": "port",
             "sample": "oracle (restated cpu-fdtd plain solver, FP64) on the "
-                      "full %d-cell grid, %d steps after %d warm-up steps, "
-                      "%.1f s" % (nx, nsteps, probe, el)}
+                      "full %d-cell grid, median of %d pieces of %d steps "
+                      "after %d warm-up steps, %.1f s in total, %d OpenMP "
+                      "threads pinned to cores of %s"
+                      % (nx, reps, per, 2 * probe, total, threads,
+                         _cpu_model())}
 
 
 def measure(specs, form, warmup, steps, opts):
@@ -245,6 +277,8 @@ def main():
                     metavar="BYTES",
                     help="only run a device-to-device copy of BYTES")
     args = ap.parse_args()
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

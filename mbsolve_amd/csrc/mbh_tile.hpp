@@ -227,7 +227,9 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
         hz[c] = in ? (k.hzero[g] != 0) : true;
         e[c] = in ? ld<ST>(k.e_in, g) : 0.0;
         h[c] = in ? ld<ST>(k.h_in, g) : 0.0;
-        p[c] = in ? ld<ST>(k.p_in, g) : 0.0;
+        /* no density: p keeps its initial zero (lindblad_noop) and is
+         * neither read nor written */
+        p[c] = (A::DC > 0 && in) ? ld<ST>(k.p_in, g) : 0.0;
 #pragma unroll
         for (int j = 0; j < A::DC; j++) {
             d[c].v[j] = in ? ld<ST>(k.d_in, (uint64_t)j * k.dstride + g) : 0.0;
@@ -395,7 +397,9 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
         if (g >= own_lo && g < own_hi) {
             st<ST>(k.e_out, g, e[c]);
             st<ST>(k.h_out, g, h[c]);
-            st<ST>(k.p_out, g, p[c]);
+            if constexpr (A::DC > 0) {
+                st<ST>(k.p_out, g, p[c]);
+            }
 #pragma unroll
             for (int j = 0; j < A::DC; j++) {
                 st<ST>(k.d_out, (uint64_t)j * k.dstride + g, d[c].v[j]);

@@ -304,6 +304,31 @@ def qcl6_sweep(num_runs=64, seed=1, min_log2=14, max_log2=18,
     return specs
 
 
+def maxwell_line(num_gridpoints=1 << 27, num_timesteps=4000,
+                 name="maxwell"):
+    """Pure Maxwell stencil (noop density): a dielectric line (eps_r 4,
+    small losses) of num_gridpoints cells with dx = 1 nm, a soft Gaussian
+    source at cell 100 and one point record of E; end time chosen for
+    num_timesteps steps."""
+    dx = 1e-9
+    L = dx * (num_gridpoints - 1)
+    v = C0 / 2.0
+    dt = 0.5 * dx / v
+    endtime = (num_timesteps - 1.5) * dt
+    return {
+        "name": name, "algo": "noop", "num_levels": 0,
+        "materials": {"Die": {"rel_permittivity": 4.0, "losses": 10.0}},
+        "regions": [("line", "Die", 0.0, L)],
+        "num_gridpoints": num_gridpoints, "endtime": endtime,
+        "rho_init": ([1.0, 0.0], []),
+        "sources": [{"kind": "gauss", "name": "g", "position": 100 * dx,
+                     "hard": False, "ampl": 1e3, "freq": 1e15,
+                     "phase": 500 * dt, "tau": 100 * dt}],
+        "records": [{"name": "e", "interval": 0.0,
+                     "position": 400 * dx}],
+    }
+
+
 # ---------------------------------------------- reference tool setups -----
 def ziolkowski1995(num_gridpoints=32768, endtime=200e-15):
     """mbsolve-tool.cpp:249-309 (self-induced transparency, digitized in

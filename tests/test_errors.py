@@ -29,7 +29,14 @@ def test_names_registered_once():
         assert "hip-fdtd-%dlvl-rodr" % n in names
         assert "hip-fdtd-%dlvl-rodr-mixed" % n in names
     assert "hip-fdtd-2lvl-reg-cayley" in names
+    assert "hip-fdtd-2lvl-reg-cayley-zform" in names
     assert "hip-fdtd-noop" in names
+    for n in range(3, 7):
+        assert "hip-fdtd-%dlvl-rodr-mfma" % n in names
+        assert "hip-fdtd-%dlvl-rodr-mfma64" % n in names
+    for bad in ("hip-fdtd-2lvl-rodr-mfma", "hip-fdtd-2lvl-reg-cayley-mfma",
+                "hip-fdtd-3lvl-reg-cayley-zform", "hip-fdtd-noop-zform"):
+        assert bad not in names
     mb.load_hip_solvers()  # a second loader is harmless
     assert mb.solver.get_avail_solvers() == names
 
@@ -147,3 +154,24 @@ def test_capi_null_arguments():
     assert b"no runs" in lib.mbh_last_error(None)
     assert lib.mbh_run(None, 0, 1) == 1
     lib.mbh_destroy(None)
+
+
+@pytest.mark.parametrize("form,code,msg", [
+    ("mfma", 4, "MFMA form"),     # Cayley has no MFMA form
+    (9, 1, "unknown form"),
+])
+def test_capi_rejects_unsupported_forms(form, code, msg):
+    d = _good_problem()
+    with pytest.raises(capi.MBHError, match=msg) as ei:
+        capi.Context([d], capi.options(form))
+    assert ei.value.code == code
+
+
+def test_capi_mfma_needs_three_levels():
+    """A two-level Rodrigues problem is refused by the MFMA form."""
+    dev, scen = setups.build(setups.sit_2lvl(num_gridpoints=64,
+                                             endtime=2e-14))
+    d = mb.build_problem("hip-fdtd-2lvl-rodr-zform", dev, scen)
+    with pytest.raises(capi.MBHError, match="MFMA form") as ei:
+        capi.Context([d], capi.options("mfma64"))
+    assert ei.value.code == 4
