@@ -738,7 +738,7 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
 {
     const uint32_t kform = kernel_form(ctx->algo, ctx->levels, ctx->form);
     std::vector<uint32_t> cs;
-    for (uint32_t C : { 1u, 2u, 4u }) {
+    for (uint32_t C : { 1u, 2u, 4u, 8u }) {
         if (find_kernel(ctx->algo, ctx->levels, kform, C) &&
             (!req_tile || req_tile == 64u * C)) {
             cs.push_back(C);

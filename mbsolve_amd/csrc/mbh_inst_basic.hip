@@ -26,6 +26,8 @@ template __global__ void tile_kernel<algo_bloch2, 4, float>(KArgs);
 template __global__ void tile_kernel<algo_bloch2, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_bloch2, 2, double>(KArgs);
 template __global__ void tile_kernel<algo_bloch2, 4, double>(KArgs);
+template __global__ void tile_kernel<algo_bloch2, 8, float>(KArgs);
+template __global__ void tile_kernel<algo_bloch2, 8, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, float>(KArgs);
 
@@ -47,6 +49,8 @@ const kernel_entry k_table_basic[] = {
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 1, algo_bloch2, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 2, algo_bloch2, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 4, algo_bloch2, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 8, algo_bloch2, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 8, algo_bloch2, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_FP64, 1, algo_cayley<3>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_MIXED, 1, algo_cayley<3>, float),
 };
