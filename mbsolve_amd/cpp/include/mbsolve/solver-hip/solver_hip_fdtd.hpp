@@ -22,7 +22,8 @@ namespace mbsolve {
 /** Parses "hip-fdtd-noop[-mixed]",
  * "hip-fdtd-<N>lvl-rodr[-mixed|-zform|-mfma|-mfma64]" (MFMA forms for
  * N >= 3), "hip-fdtd-2lvl-reg-cayley[-mixed|-zform]" and
- * "hip-fdtd-3lvl-reg-cayley[-mixed]"; returns false if not a HIP name. */
+ * "hip-fdtd-<N>lvl-reg-cayley[-mixed]" (N = 3..6); returns false if not a
+ * HIP name. */
 bool parse_hip_solver_name(const std::string& name, density_algo& algo,
                            unsigned int& num_lvl, uint32_t& form);
 

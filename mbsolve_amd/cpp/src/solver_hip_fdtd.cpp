@@ -95,7 +95,7 @@ parse_hip_solver_name(const std::string& name_in, density_algo& algo,
         algo = density_algo::cvr_rodr;
         return !mfma || num_lvl >= 3;
     }
-    if (alg == "reg-cayley" && (num_lvl == 2 || num_lvl == 3)) {
+    if (alg == "reg-cayley") {
         algo = density_algo::reg_cayley;
         return !mfma && (num_lvl == 2 || form != MBH_FORM_ZFORM64);
     }
@@ -288,8 +288,11 @@ solver_hip_loader::solver_hip_loader()
         for (const char* f : { "", "-mixed", "-zform" }) {
             names.push_back(std::string("hip-fdtd-2lvl-reg-cayley") + f);
         }
-        for (const char* f : { "", "-mixed" }) {
-            names.push_back(std::string("hip-fdtd-3lvl-reg-cayley") + f);
+        for (unsigned int n = 3; n <= 6; n++) {
+            for (const char* f : { "", "-mixed" }) {
+                names.push_back("hip-fdtd-" + std::to_string(n) +
+                                "lvl-reg-cayley" + f);
+            }
         }
         for (const auto& nm : names) {
             solver::register_bootstrap(

@@ -30,6 +30,12 @@ template __global__ void tile_kernel<algo_bloch2, 8, float>(KArgs);
 template __global__ void tile_kernel<algo_bloch2, 8, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, float>(KArgs);
+template __global__ void tile_kernel<algo_cayley<4>, 1, double>(KArgs);
+template __global__ void tile_kernel<algo_cayley<4>, 1, float>(KArgs);
+template __global__ void tile_kernel<algo_cayley<5>, 1, double>(KArgs);
+template __global__ void tile_kernel<algo_cayley<5>, 1, float>(KArgs);
+template __global__ void tile_kernel<algo_cayley<6>, 1, double>(KArgs);
+template __global__ void tile_kernel<algo_cayley<6>, 1, float>(KArgs);
 
 #if !defined(__HIP_DEVICE_COMPILE__)
 
@@ -53,6 +59,12 @@ const kernel_entry k_table_basic[] = {
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 8, algo_bloch2, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_FP64, 1, algo_cayley<3>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_MIXED, 1, algo_cayley<3>, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 4, MBH_FORM_FP64, 1, algo_cayley<4>, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 4, MBH_FORM_MIXED, 1, algo_cayley<4>, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 5, MBH_FORM_FP64, 1, algo_cayley<5>, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 5, MBH_FORM_MIXED, 1, algo_cayley<5>, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 6, MBH_FORM_FP64, 1, algo_cayley<6>, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 6, MBH_FORM_MIXED, 1, algo_cayley<6>, float),
 };
 const uint32_t k_table_basic_n = sizeof(k_table_basic) / sizeof(k_table_basic[0]);
 #endif

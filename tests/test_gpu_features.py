@@ -127,6 +127,19 @@ def test_more_levels_fp64(gpu, make):
         assert rel_err(r.get_data_real(), b) <= 1e-12, r.get_name()
 
 
+@pytest.mark.parametrize("levels", [4, 5, 6])
+def test_cayley_many_levels_fp64(gpu, levels):
+    """Regularized Cayley for 4..6 levels (the reference's
+    cpu-fdtd-<N>lvl-reg-cayley): the anharmonic ladder against the
+    oracle's reference-order update."""
+    spec = setups.marskar2011(levels, num_gridpoints=3000, endtime=0.2e-12)
+    spec["algo"] = "reg-cayley"
+    sol = _solve(spec)
+    ref = oracle.OracleRun(sol.problem()).run().records()
+    for r, b in zip(sol.get_results(), ref):
+        assert rel_err(r.get_data_real(), b) <= 1e-12, r.get_name()
+
+
 def test_noop_maxwell(gpu):
     spec = {"name": "vac", "algo": "noop", "num_levels": 0,
             "materials": {"Vacuum": {}, "Die": {"rel_permittivity": 4.0}},
