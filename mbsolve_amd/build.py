@@ -50,8 +50,9 @@ def _run(cmd):
 
 def _hip_object(src):
     obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-    _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(PKG, "csrc", src),
-                                "-o", obj])
+    extra = os.environ.get("MBSOLVE_HIP_EXTRA_FLAGS", "").split()
+    _run([HIPCC] + HIP_FLAGS + extra +
+         ["-c", os.path.join(PKG, "csrc", src), "-o", obj])
     return obj
 
 

@@ -257,8 +257,17 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
     }
 }
 
+/* ZM_WAVES_PER_EU (build option) asks the compiler for that many
+ * wavefronts per SIMD, spilling if it has to */
+#ifdef ZM_WAVES_PER_EU
+#define ZM_OCCUPANCY \
+    __attribute__((amdgpu_waves_per_eu(ZM_WAVES_PER_EU, ZM_WAVES_PER_EU)))
+#else
+#define ZM_OCCUPANCY
+#endif
+
 template<int N, typename ST>
-__global__ __launch_bounds__(64) void zm_kernel(KArgs k)
+__global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
 {
     using Gm = zm_geom<N>;
     __shared__ double s_src[TILE_SOURCES][MAX_BLOCK];
