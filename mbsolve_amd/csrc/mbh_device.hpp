@@ -165,6 +165,7 @@ __device__ __forceinline__ void cinverse(const cplx* A, cplx* R)
  * lindblad_noop (algo_lindblad_noop.hpp:35-86): no density, p untouched.
  * ==================================================================== */
 struct algo_noop {
+    static constexpr int STAGE = 0;
     static constexpr bool FAST_FDTD = false;
     static constexpr int DC = 0; /* stored density components */
     static constexpr int STATE = 1;
@@ -193,6 +194,7 @@ struct algo_noop {
  * ==================================================================== */
 template<int N>
 struct algo_cayley {
+    static constexpr int STAGE = 0;
     static constexpr bool FAST_FDTD = false;
     static constexpr int DC = 2 * N * N;
     static constexpr int STATE = 2 * N * N;
@@ -362,6 +364,7 @@ struct algo_cayley {
  *   [22..25] D1 (row-major)  [26] g  [27] 0
  * ==================================================================== */
 struct algo_bloch2 {
+    static constexpr int STAGE = 0;
     static constexpr int DC = 4;
     static constexpr int STATE = 4;
     static constexpr bool FAST_FDTD = true;
@@ -490,6 +493,7 @@ __device__ __forceinline__ double cv_population(const double* d, int m)
  * ==================================================================== */
 template<int N>
 struct algo_rodr {
+    static constexpr int STAGE = 0;
     static constexpr bool FAST_FDTD = false;
     static constexpr int K = N * N - 1;
     static constexpr int B = K / 2;
@@ -716,17 +720,29 @@ struct algo_zform {
 
     __device__ static bool has_qm(cdp q) { return q[0] != 0.0; }
 
-    /* C cells of one material: straight-line code with the cell index
-     * innermost, so the independent chains interleave.  Planes beyond
-     * n_theta have angle 0 (identity rotation, exact). */
+    /* Small blobs (K <= 8) are staged in LDS by the tile kernel: W, c, a
+     * and theta are then read from there every step instead of through
+     * the scalar cache (their 600 bytes do not fit the SGPRs). */
+    static constexpr int STAGE = (K <= 8) ? OFF_A + K : 0;
+
     template<int C>
     __device__ static void update_multi(cdp q, state* s, const double* e,
                                         double* p)
     {
-        cdp th = q + OFF_TH;
-        cdp W = q + OFF_W;
-        cdp cv = q + OFF_C;
-        cdp a = q + OFF_A;
+        update_multi_p<C>(q, s, e, p);
+    }
+
+    /* C cells of one material: straight-line code with the cell index
+     * innermost, so the independent chains interleave.  Planes beyond
+     * n_theta have angle 0 (identity rotation, exact). */
+    template<int C, typename P>
+    __device__ static void update_multi_p(P q, state* s, const double* e,
+                                          double* p)
+    {
+        const P th = q + OFF_TH;
+        const P W = q + OFF_W;
+        const P cv = q + OFF_C;
+        const P a = q + OFF_A;
 
         double sn[C][B], cs[C][B];
         bool small = true;

@@ -267,6 +267,17 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
     const bool uniform = __all(same);
     cdp q0 = (cdp)k.qm + (uint64_t)(m0 < 0 ? 0 : m0) * k.qstride;
     const bool q0_active = uniform && m0 >= 0 && A::DC > 0 && A::has_qm(q0);
+    /* doubles of the material blob the algorithm wants staged in LDS */
+    constexpr int STAGE = A::STAGE;
+    __shared__ double s_q[STAGE > 0 ? STAGE : 1];
+    if constexpr (STAGE > 0) {
+        if (q0_active) {
+            for (int i = lane; i < STAGE; i += 64) {
+                s_q[i] = q0[i];
+            }
+        }
+        __syncthreads();
+    }
 
     /* sources of this tile: their values for the block's steps go to LDS */
     const uint32_t s_lo = as_const(k.tile_src)[2 * t];
@@ -349,7 +360,17 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
         }
         if constexpr (A::DC > 0) {
             if (q0_active) {
-                A::template update_multi<C>(q0, d, e, p);
+                if constexpr (STAGE > 0) {
+                    /* an offset the compiler cannot see through keeps the
+                     * LDS reads inside the step (hoisted, the staged
+                     * constants would occupy VGPRs for the whole loop) */
+                    int off = 0;
+                    asm volatile("" : "+s"(off));
+                    A::template update_multi_p<C>((const double*)s_q + off,
+                                                  d, e, p);
+                } else {
+                    A::template update_multi<C>(q0, d, e, p);
+                }
             } else if (!uniform) {
 #pragma unroll
                 for (int c = 0; c < C; c++) {
