@@ -14,6 +14,7 @@
 #define MBH_DEVICE_HPP
 
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 
 #include "../../include/mbsolve_hip.h"
@@ -682,10 +683,25 @@ __device__ __forceinline__ void taylor_sincos(double x, double& s, double& c)
     c = fma(pc, x2, 1.0);
 }
 
-/* Taylor series for |x| <= TAYLOR_MAX, library sincos otherwise. */
-__device__ __forceinline__ void zsincos(double x, double& s, double& c)
+
+/* sin and cos of a small angle x, by wave-uniform tier of max |x|:
+ * TIER 0: |x| <= 2^-13, x - x^3/6 and 1 - x^2/2 (relative truncation
+ * error below 1e-17); TIER 1: |x| <= 2^-10, Taylor to x^5 / x^4 (below
+ * 2e-21 / 3e-19); TIER 2: |x| <= 0.25, Taylor to x^13 / x^14; TIER 3:
+ * library. */
+template<int TIER>
+__device__ __forceinline__ void tier_sincos(double x, double& s, double& c)
 {
-    if (fabs(x) <= TAYLOR_MAX) {
+    if constexpr (TIER == 0) {
+        const double x2 = x * x;
+        s = fma(x * x2, -1.6666666666666666667e-01, x);
+        c = fma(x2, -0.5, 1.0);
+    } else if constexpr (TIER == 1) {
+        const double x2 = x * x;
+        s = fma(x * x2, fma(x2, 8.3333333333333333333e-03,
+                            -1.6666666666666666667e-01), x);
+        c = fma(x2, fma(x2, 4.1666666666666666667e-02, -0.5), 1.0);
+    } else if constexpr (TIER == 2) {
         taylor_sincos(x, s, c);
     } else {
         sincos(x, &s, &c);
@@ -745,32 +761,39 @@ struct algo_zform {
         const P a = q + OFF_A;
 
         double sn[C][B], cs[C][B];
-        bool small = true;
+        bool t0 = true, t1 = true, t2 = true;
 #pragma unroll
         for (int c = 0; c < C; c++) {
 #pragma unroll
             for (int b = 0; b < B; b++) {
                 const double x = th[b] * e[c];
+                const double ax = fabs(x);
                 sn[c][b] = x;
-                small = small && (fabs(x) <= TAYLOR_MAX);
+                t0 = t0 && (ax <= 0x1p-13);
+                t1 = t1 && (ax <= 0x1p-10);
+                t2 = t2 && (ax <= TAYLOR_MAX);
             }
         }
-        if (__all(small)) {
+        /* wave-uniform tier: the cheapest series accurate for every
+         * angle of the wavefront */
+        auto all_sincos = [&](auto tier) {
+            constexpr int T = decltype(tier)::value;
 #pragma unroll
             for (int c = 0; c < C; c++) {
 #pragma unroll
                 for (int b = 0; b < B; b++) {
-                    taylor_sincos(sn[c][b], sn[c][b], cs[c][b]);
+                    tier_sincos<T>(sn[c][b], sn[c][b], cs[c][b]);
                 }
             }
+        };
+        if (__all(t0)) {
+            all_sincos(std::integral_constant<int, 0>());
+        } else if (__all(t1)) {
+            all_sincos(std::integral_constant<int, 1>());
+        } else if (__all(t2)) {
+            all_sincos(std::integral_constant<int, 2>());
         } else {
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-#pragma unroll
-                for (int b = 0; b < B; b++) {
-                    zsincos(sn[c][b], sn[c][b], cs[c][b]);
-                }
-            }
+            all_sincos(std::integral_constant<int, 3>());
         }
 
         /* r = R(e) y */
@@ -788,23 +811,29 @@ struct algo_zform {
                 r[c][2 * b + 1] = fma(sn[c][b], y0, cs[c][b] * y1);
             }
         }
-        /* p = a . r + b */
-        double pa[C];
+        /* p = a . r + b, two interleaved partial sums (shorter chain on
+         * the step's critical path) */
+        double pa[C], pb[C];
 #pragma unroll
         for (int c = 0; c < C; c++) {
             pa[c] = q[2];
+            pb[c] = 0.0;
         }
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const double ak = a[k];
 #pragma unroll
             for (int c = 0; c < C; c++) {
-                pa[c] = fma(ak, r[c][k], pa[c]);
+                if (k & 1) {
+                    pb[c] = fma(ak, r[c][k], pb[c]);
+                } else {
+                    pa[c] = fma(ak, r[c][k], pa[c]);
+                }
             }
         }
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            p[c] = pa[c];
+            p[c] = pa[c] + pb[c];
         }
         /* y' = W r + c */
 #pragma unroll

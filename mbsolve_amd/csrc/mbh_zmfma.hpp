@@ -93,30 +93,6 @@ struct zm_geom {
     }
 };
 
-/* sin and cos of a small angle x, by wave-uniform tier of max |x|:
- * TIER 0: |x| <= 2^-13, x - x^3/6 and 1 - x^2/2 (relative truncation
- * error below 1e-17); TIER 1: |x| <= 2^-10, Taylor to x^5 / x^4 (below
- * 2e-21 / 3e-19); TIER 2: |x| <= 0.25, Taylor to x^13 / x^14; TIER 3:
- * library. */
-template<int TIER>
-__device__ __forceinline__ void tier_sincos(double x, double& s, double& c)
-{
-    if constexpr (TIER == 0) {
-        const double x2 = x * x;
-        s = fma(x * x2, -1.6666666666666666667e-01, x);
-        c = fma(x2, -0.5, 1.0);
-    } else if constexpr (TIER == 1) {
-        const double x2 = x * x;
-        s = fma(x * x2, fma(x2, 8.3333333333333333333e-03,
-                            -1.6666666666666666667e-01), x);
-        c = fma(x2, fma(x2, 4.1666666666666666667e-02, -0.5), 1.0);
-    } else if constexpr (TIER == 2) {
-        taylor_sincos(x, s, c);
-    } else {
-        sincos(x, &s, &c);
-    }
-}
-
 /* Lane (q, j) holds v[n] for blocks n = 0..3; returns sum over the four
  * lane groups of v[q] (block n = q, i.e. the cell of this lane in the
  * one-cell-per-lane layout).  Two exchange rounds (xor 32, xor 16). */
