@@ -217,3 +217,34 @@ def test_hdf5_writer_after_run(gpu, tmp_path):
     for r in sol.get_results():
         assert np.array_equal(f["results"][r.get_name()]["real"],
                               r.get_data_real())
+
+
+@pytest.mark.parametrize("name", ["hip-fdtd-2lvl-reg-cayley",
+                                  "hip-fdtd-3lvl-rodr-zform"])
+def test_graph_replays_match_direct_launches(gpu, name):
+    """Blocks of launches captured once in a hipGraph and replayed many
+    times, with fetches (synchronisations, host work) between the runs:
+    the state after every piece and the records equal those of direct
+    launches exactly."""
+    spec = (setups.sit_2lvl(num_gridpoints=1000, endtime=0.3e-12,
+                            num_records=8)
+            if "cayley" in name else
+            setups.v3_rodr(num_gridpoints=1000, endtime=0.3e-12,
+                           num_records=8))
+    sols = []
+    for graph in (1, 4):
+        dev, scen = setups.build(spec)
+        sols.append(mb.create_hip_solver(name, dev, scen, temporal_block=4,
+                                         tile_cells=64,
+                                         graph_launches=graph))
+    assert sols[1].info()["graph_launches"] == 4
+    nt = sols[0].problem()["num_timesteps"]
+    bounds = list(range(0, nt, 50)) + [nt]
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        for sol in sols:
+            sol.run_steps(a, b)
+        s0, s1 = sols[0].fetch_state(), sols[1].fetch_state()
+        for key in ("e", "h", "p", "d"):
+            assert np.array_equal(s0[key], s1[key]), (a, b, key)
+    for r0, r1 in zip(sols[0].get_results(), sols[1].get_results()):
+        assert np.array_equal(r0.get_data_real(), r1.get_data_real())
