@@ -802,16 +802,23 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
         HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
         float ms0 = 0.0f;
         HIP_TRY(ctx, hipEventElapsedTime(&ms0, ctx->ev0, ctx->ev1));
-        const int reps = (ms0 > 20.0f) ? 1 : 4;
-        HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-        for (int i = 0; i < reps; i++) {
-            HIP_TRY(ctx, launch_block(ctx, k));
+        /* about 3 ms of launches per measurement, the faster of two */
+        const int reps = std::max(
+            1, std::min(64, (int)std::ceil(3.0 / std::max(ms0, 1e-3f))));
+        float ms = 1e30f;
+        for (int round = 0; round < 2; round++) {
+            HIP_TRY(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+            for (int i = 0; i < reps; i++) {
+                HIP_TRY(ctx, launch_block(ctx, k));
+            }
+            HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+            HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
+            float mr = 0.0f;
+            HIP_TRY(ctx, hipEventElapsedTime(&mr, ctx->ev0, ctx->ev1));
+            spent_ms += mr;
+            ms = std::min(ms, mr);
         }
-        HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-        HIP_TRY(ctx, hipEventSynchronize(ctx->ev1));
-        float ms = 0.0f;
-        HIP_TRY(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        spent_ms += ms0 + ms;
+        spent_ms += ms0;
         tried++;
         const double per_step = (double)ms / (reps * (double)steps);
         if (per_step < best) {
