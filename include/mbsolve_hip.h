@@ -90,6 +90,18 @@ enum {
     MBH_FORM_MFMA64 = 4
 };
 
+/* ---- step orders ------------------------------------------------------- */
+enum {
+    /* solver_cpu_fdtd::run (solver_cpu_fdtd.tpp:230-325): E update,
+     * sources, H update and boundary condition, density update, records */
+    MBH_ORDER_PLAIN = 0,
+    /* solver_cpu_fdtd_red::run (solver_cpu_fdtd_red.tpp:451-549): density
+     * update, E update, sources, H update and boundary condition, records.
+     * The density sees the field of the previous step, so inversion and
+     * density records trail the plain order by one step.  FP64 form only. */
+    MBH_ORDER_RED = 1
+};
+
 /* ---- record / source types (same values as mbsolve::record::type and
  *      mbsolve::source::type) --------------------------------------------- */
 enum {
@@ -221,7 +233,8 @@ typedef struct mbh_options {
     int32_t device;          /* HIP device ordinal, -1 = current */
     uint32_t graph_launches; /* kernel launches per captured hipGraph,
                                 0 = auto, 1 = no graph */
-    uint32_t reserved[3];
+    uint32_t step_order;     /* MBH_ORDER_* */
+    uint32_t reserved[2];
 } mbh_options;
 
 typedef struct mbh_info {
@@ -234,7 +247,8 @@ typedef struct mbh_info {
     uint64_t launches;     /* kernel launches issued by the last mbh_run */
     uint64_t state_bytes;  /* device bytes of one state buffer */
     uint32_t graph_launches;
-    uint32_t reserved[3];
+    uint32_t step_order;   /* MBH_ORDER_* */
+    uint32_t reserved[2];
 } mbh_info;
 
 typedef struct mbh_ctx mbh_ctx;
@@ -245,7 +259,8 @@ const char* mbh_version(void);
 /* Number of visible HIP devices (0 when none). */
 int mbh_device_count(int* count);
 
-/* Default options: MBH_FORM_FP64, automatic tiling, current device. */
+/* Default options: MBH_FORM_FP64, plain step order, automatic tiling,
+ * current device. */
 void mbh_default_options(mbh_options* opt);
 
 /* Uploads constants, initial state, source tables and record descriptors
@@ -257,9 +272,10 @@ int mbh_create(const mbh_problem* runs,
                const mbh_options* opt,
                mbh_ctx** out);
 
-/* Advances all runs through time steps n in [n_begin, n_end), in the plain
- * step order of solver_cpu_fdtd::run.  Returns once the work is queued on
- * the context's stream; use mbh_synchronize or a fetch to wait. */
+/* Advances all runs through time steps n in [n_begin, n_end), in the step
+ * order chosen at creation (solver_cpu_fdtd::run or solver_cpu_fdtd_red::run,
+ * solver_cpu_fdtd_red.tpp:351-567).  Returns once the work is queued on the
+ * context's stream; use mbh_synchronize or a fetch to wait. */
 int mbh_run(mbh_ctx* ctx, uint64_t n_begin, uint64_t n_end);
 
 /* Waits for all queued work of the context. */

@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 HIP_SOURCES = ["mbh_capi.hip", "mbh_inst_basic.hip", "mbh_inst_rodr.hip",
-               "mbh_inst_zform.hip", "mbh_inst_zmfma.hip"]
+               "mbh_inst_zform.hip", "mbh_inst_zmfma.hip", "mbh_inst_red.hip"]
 CPP_SOURCES = ["lib_model.cpp", "numerics.cpp", "cv_representation.cpp",
                "fdtd_setup.cpp", "solver_hip_fdtd.cpp"]
 

@@ -19,6 +19,8 @@ MBH_OK = 0
 MBH_ALGO_NOOP, MBH_ALGO_REG_CAYLEY, MBH_ALGO_CVR_RODR = 0, 1, 2
 MBH_FORM_FP64, MBH_FORM_MIXED, MBH_FORM_ZFORM64 = 0, 1, 2
 MBH_FORM_MFMA, MBH_FORM_MFMA64 = 3, 4
+MBH_ORDER_PLAIN, MBH_ORDER_RED = 0, 1
+ORDERS = {"plain": MBH_ORDER_PLAIN, "red": MBH_ORDER_RED}
 FORMS = {"fp64": MBH_FORM_FP64, "mixed": MBH_FORM_MIXED,
          "zform64": MBH_FORM_ZFORM64, "mfma": MBH_FORM_MFMA,
          "mfma64": MBH_FORM_MFMA64}
@@ -64,8 +66,8 @@ class mbh_problem(C.Structure):
 class mbh_options(C.Structure):
     _fields_ = [("form", C.c_uint32), ("temporal_block", C.c_uint32),
                 ("tile_cells", C.c_uint32), ("device", C.c_int32),
-                ("graph_launches", C.c_uint32),
-                ("reserved", C.c_uint32 * 3)]
+                ("graph_launches", C.c_uint32), ("step_order", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class mbh_info(C.Structure):
@@ -73,7 +75,8 @@ class mbh_info(C.Structure):
                 ("tile_cells", C.c_uint32), ("owned_cells", C.c_uint32),
                 ("num_tiles", C.c_uint64), ("total_cells", C.c_uint64),
                 ("launches", C.c_uint64), ("state_bytes", C.c_uint64),
-                ("graph_launches", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("graph_launches", C.c_uint32), ("step_order", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 _lib = None
@@ -188,7 +191,7 @@ def problem_from_dict(d):
 
 
 def options(form="fp64", temporal_block=0, tile_cells=0, graph_launches=0,
-            device=-1):
+            device=-1, order="plain"):
     lib = load()
     o = mbh_options()
     lib.mbh_default_options(C.byref(o))
@@ -197,6 +200,7 @@ def options(form="fp64", temporal_block=0, tile_cells=0, graph_launches=0,
     o.tile_cells = tile_cells
     o.graph_launches = graph_launches
     o.device = device
+    o.step_order = ORDERS[order] if isinstance(order, str) else int(order)
     return o
 
 

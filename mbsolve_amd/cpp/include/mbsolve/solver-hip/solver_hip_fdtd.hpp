@@ -21,11 +21,16 @@ namespace mbsolve {
 
 /** Parses "hip-fdtd-noop[-mixed]",
  * "hip-fdtd-<N>lvl-rodr[-mixed|-zform|-mfma|-mfma64]" (MFMA forms for
- * N >= 3), "hip-fdtd-2lvl-reg-cayley[-mixed|-zform]" and
- * "hip-fdtd-<N>lvl-reg-cayley[-mixed]" (N = 3..6); returns false if not a
+ * N >= 3), "hip-fdtd-2lvl-reg-cayley[-mixed|-zform]",
+ * "hip-fdtd-<N>lvl-reg-cayley[-mixed]" (N = 3..6) and the red step order
+ * "hip-fdtd-red-noop", "hip-fdtd-red-<N>lvl-rodr",
+ * "hip-fdtd-red-<N>lvl-reg-cayley" (FP64 form); returns false if not a
  * HIP name. */
 bool parse_hip_solver_name(const std::string& name, density_algo& algo,
                            unsigned int& num_lvl, uint32_t& form);
+
+/** MBH_ORDER_RED for "hip-fdtd-red-*" names, else MBH_ORDER_PLAIN. */
+uint32_t hip_solver_order(const std::string& name);
 
 /** Launch options from the environment: MBSOLVE_HIP_TEMPORAL_BLOCK,
  * MBSOLVE_HIP_TILE_CELLS, MBSOLVE_HIP_GRAPH_LAUNCHES, MBSOLVE_HIP_DEVICE. */

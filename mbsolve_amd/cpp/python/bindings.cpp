@@ -117,6 +117,7 @@ py::dict info_dict(const mbh_info& i)
     d["launches"] = i.launches;
     d["state_bytes"] = i.state_bytes;
     d["graph_launches"] = i.graph_launches;
+    d["step_order"] = i.step_order;
     return d;
 }
 

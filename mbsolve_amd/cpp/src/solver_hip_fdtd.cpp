@@ -58,6 +58,11 @@ parse_hip_solver_name(const std::string& name_in, density_algo& algo,
         return false;
     }
     std::string rest = name_in.substr(pre.size());
+    /* "red-": the step order of solver_cpu_fdtd_red (FP64 form only) */
+    const bool red = rest.compare(0, 4, "red-") == 0;
+    if (red) {
+        rest = rest.substr(4);
+    }
     form = MBH_FORM_FP64;
     auto strip = [&rest](const std::string& suf) {
         if (rest.size() > suf.size() &&
@@ -77,6 +82,9 @@ parse_hip_solver_name(const std::string& name_in, density_algo& algo,
         form = MBH_FORM_MFMA;
     }
     const bool mfma = (form == MBH_FORM_MFMA || form == MBH_FORM_MFMA64);
+    if (red && form != MBH_FORM_FP64) {
+        return false;
+    }
     if (rest == "noop") {
         algo = density_algo::noop;
         num_lvl = 0;
@@ -100,6 +108,13 @@ parse_hip_solver_name(const std::string& name_in, density_algo& algo,
         return !mfma && (num_lvl == 2 || form != MBH_FORM_ZFORM64);
     }
     return false;
+}
+
+uint32_t
+hip_solver_order(const std::string& name)
+{
+    return name.compare(0, 13, "hip-fdtd-red-") == 0 ? MBH_ORDER_RED
+                                                     : MBH_ORDER_PLAIN;
 }
 
 mbh_options
@@ -137,6 +152,7 @@ solver_hip_fdtd::solver_hip_fdtd(std::shared_ptr<const device> dev,
     }
     mbh_options o = opt ? *opt : hip_options_from_env(m_form);
     o.form = m_form;
+    o.step_order = hip_solver_order(name);
     check(mbh_create(&m_problem.prob, 1, &o, &m_ctx), nullptr, "mbh_create");
 }
 
@@ -213,6 +229,7 @@ solver_hip_batch::solver_hip_batch(
     }
     mbh_options o = opt ? *opt : hip_options_from_env(form);
     o.form = form;
+    o.step_order = hip_solver_order(name);
     check(mbh_create(probs.data(), (uint32_t)probs.size(), &o, &m_ctx),
           nullptr, "mbh_create");
 }
@@ -293,6 +310,14 @@ solver_hip_loader::solver_hip_loader()
                 names.push_back("hip-fdtd-" + std::to_string(n) +
                                 "lvl-reg-cayley" + f);
             }
+        }
+        /* step order of solver_cpu_fdtd_red (solver_cpu_loader.cpp:38-40,
+         * 54-64, 78-88) */
+        names.push_back("hip-fdtd-red-noop");
+        for (unsigned int n = 2; n <= 6; n++) {
+            names.push_back("hip-fdtd-red-" + std::to_string(n) + "lvl-rodr");
+            names.push_back("hip-fdtd-red-" + std::to_string(n) +
+                            "lvl-reg-cayley");
         }
         for (const auto& nm : names) {
             solver::register_bootstrap(

@@ -312,6 +312,7 @@ struct mbh_ctx {
     bool timed = false;
 
     uint32_t algo = 0, levels = 0, form = 0;
+    uint32_t order = MBH_ORDER_PLAIN;
     uint64_t n_t = 0;
     uint32_t DC = 0;        /* stored density components per cell */
     uint32_t dref = 0;      /* reference density components per cell */
@@ -586,17 +587,17 @@ int validate(const mbh_problem* p, uint32_t r, const mbh_problem* p0,
 }
 
 const kernel_entry* find_kernel(uint32_t algo, uint32_t levels, uint32_t form,
-                                uint32_t C)
+                                uint32_t C, uint32_t order)
 {
-    const kernel_entry* tabs[4] = { k_table_basic, k_table_rodr,
-                                    k_table_zform, k_table_zm };
-    const uint32_t ns[4] = { k_table_basic_n, k_table_rodr_n,
-                             k_table_zform_n, k_table_zm_n };
-    for (int t = 0; t < 4; t++) {
+    const kernel_entry* tabs[5] = { k_table_basic, k_table_rodr,
+                                    k_table_zform, k_table_zm, k_table_red };
+    const uint32_t ns[5] = { k_table_basic_n, k_table_rodr_n,
+                             k_table_zform_n, k_table_zm_n, k_table_red_n };
+    for (int t = 0; t < 5; t++) {
         for (uint32_t i = 0; i < ns[t]; i++) {
             const kernel_entry& k = tabs[t][i];
             if (k.algo == algo && k.levels == levels && k.form == form &&
-                k.cells_per_lane == C) {
+                k.cells_per_lane == C && k.order == order) {
                 return &k;
             }
         }
@@ -739,7 +740,7 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
     const uint32_t kform = kernel_form(ctx->algo, ctx->levels, ctx->form);
     std::vector<uint32_t> cs;
     for (uint32_t C : { 1u, 2u, 4u, 8u }) {
-        if (find_kernel(ctx->algo, ctx->levels, kform, C) &&
+        if (find_kernel(ctx->algo, ctx->levels, kform, C, ctx->order) &&
             (!req_tile || req_tile == 64u * C)) {
             cs.push_back(C);
         }
@@ -772,7 +773,8 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
     if (cand.size() == 1) {
         set_geometry(ctx, cand[0].first, cand[0].second);
         ctx->launch =
-            find_kernel(ctx->algo, ctx->levels, kform, ctx->C)->fn;
+            find_kernel(ctx->algo, ctx->levels, kform, ctx->C, ctx->order)
+                ->fn;
         return MBH_OK;
     }
     /* cheapest candidates (shallow blocks) first would bias the budget;
@@ -787,7 +789,9 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
             break;
         }
         set_geometry(ctx, cd.first, cd.second);
-        ctx->launch = find_kernel(ctx->algo, ctx->levels, kform, cd.first)->fn;
+        ctx->launch =
+            find_kernel(ctx->algo, ctx->levels, kform, cd.first, ctx->order)
+                ->fn;
         int rc = build_tile_tables(ctx, false);
         if (rc) {
             return rc;
@@ -828,7 +832,8 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
     }
     ctx->tuned_candidates = tried;
     set_geometry(ctx, pick.first, pick.second);
-    ctx->launch = find_kernel(ctx->algo, ctx->levels, kform, ctx->C)->fn;
+    ctx->launch =
+        find_kernel(ctx->algo, ctx->levels, kform, ctx->C, ctx->order)->fn;
     return MBH_OK;
 }
 
@@ -962,6 +967,15 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     if (opt.form > MBH_FORM_MFMA64) {
         return fail(nullptr, MBH_ERR_INVALID, "unknown form %u", opt.form);
     }
+    if (opt.step_order > MBH_ORDER_RED) {
+        return fail(nullptr, MBH_ERR_INVALID, "unknown step order %u",
+                    opt.step_order);
+    }
+    if (opt.step_order == MBH_ORDER_RED && opt.form != MBH_FORM_FP64) {
+        return fail(nullptr, MBH_ERR_UNSUPPORTED,
+                    "the red step order is implemented for the FP64 form "
+                    "only (form %u)", opt.form);
+    }
     for (uint32_t r = 0; r < num_runs; r++) {
         int rc = validate(&runs[r], r, &runs[0], opt.form);
         if (rc) {
@@ -990,6 +1004,7 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     c->algo = p0.algo;
     c->levels = (p0.algo == MBH_ALGO_NOOP) ? 0 : p0.num_levels;
     c->form = opt.form;
+    c->order = opt.step_order;
     c->n_t = p0.num_timesteps;
     c->dref = ref_density_stride(c->algo, c->levels);
     c->DC = c->dref;
@@ -1063,7 +1078,7 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     c->graph_launches =
         (opt.graph_launches > 1) ? ((opt.graph_launches + 1u) & ~1u) : 1u;
     if (!find_kernel(c->algo, c->levels,
-                     kernel_form(c->algo, c->levels, c->form), 1)) {
+                     kernel_form(c->algo, c->levels, c->form), 1, c->order)) {
         return fail(nullptr, MBH_ERR_UNSUPPORTED,
                     "no kernel for algo %u, %u levels, form %u", c->algo,
                     c->levels, c->form);
@@ -1495,6 +1510,7 @@ int mbh_get_info(const mbh_ctx* ctx, mbh_info* info)
     info->launches = ctx->last_launches;
     info->state_bytes = ctx->state_bytes;
     info->graph_launches = ctx->graph_launches;
+    info->step_order = ctx->order;
     return MBH_OK;
 }
 

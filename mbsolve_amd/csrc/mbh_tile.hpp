@@ -18,7 +18,11 @@
  *
  * Step order inside the block is the plain one of solver_cpu_fdtd::run
  * (solver_cpu_fdtd.tpp:230-325): E update, sources, H update with
- * boundary condition, density update, record copy.
+ * boundary condition, density update, record copy; with RED that of
+ * solver_cpu_fdtd_red::run (solver_cpu_fdtd_red.tpp:451-549): density
+ * update, E update, sources, H update with boundary condition, record copy.
+ * Both orders have the same one-cell stencil reach per step, so the halo
+ * rule above holds for either.
  */
 #ifndef MBH_TILE_HPP
 #define MBH_TILE_HPP
@@ -199,7 +203,7 @@ constexpr int TILE_SOURCES = 8;
 constexpr int MAX_BLOCK = 64; /* largest temporal block */
 
 /* One tile t: all steps of the temporal block for the tile's cells. */
-template<class A, int C, typename ST>
+template<class A, int C, typename ST, bool RED = false>
 __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 {
     __shared__ double s_src[TILE_SOURCES][MAX_BLOCK];
@@ -304,8 +308,42 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
                               record_now(ratio, rows, n0 + lane));
     }
 
+    /* density update of every cell from its current field
+     * (solver_cpu_fdtd.tpp:278-285, solver_cpu_fdtd_red.tpp:451-457) */
+    auto density_update = [&]() {
+        if constexpr (A::DC > 0) {
+            if (q0_active) {
+                if constexpr (STAGE > 0) {
+                    /* an offset the compiler cannot see through keeps the
+                     * LDS reads inside the step (hoisted, the staged
+                     * constants would occupy VGPRs for the whole loop) */
+                    int off = 0;
+                    asm volatile("" : "+s"(off));
+                    A::template update_multi_p<C>((const double*)s_q + off,
+                                                  d, e, p);
+                } else {
+                    A::template update_multi<C>(q0, d, e, p);
+                }
+            } else if (!uniform) {
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    for_each_material(m[c], [&](int32_t mm) {
+                        cdp q = (cdp)k.qm + (uint64_t)mm * k.qstride;
+                        if (A::has_qm(q)) {
+                            A::update(q, d[c], e[c], p[c]);
+                        }
+                    });
+                }
+            }
+        }
+    };
+
     for (uint32_t s = 0; s < nsteps; s++) {
         const uint64_t n = n0 + s;
+
+        if constexpr (RED) {
+            density_update();
+        }
 
         /* E update (solver_cpu_fdtd.tpp:238-243) */
         const double h_right = from_right(h[0]);
@@ -353,35 +391,12 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
             }
         }
 
-        /* density update (solver_cpu_fdtd.tpp:278-285) */
 #pragma unroll
         for (int c = 0; c < C; c++) {
             e[c] = en[c];
         }
-        if constexpr (A::DC > 0) {
-            if (q0_active) {
-                if constexpr (STAGE > 0) {
-                    /* an offset the compiler cannot see through keeps the
-                     * LDS reads inside the step (hoisted, the staged
-                     * constants would occupy VGPRs for the whole loop) */
-                    int off = 0;
-                    asm volatile("" : "+s"(off));
-                    A::template update_multi_p<C>((const double*)s_q + off,
-                                                  d, e, p);
-                } else {
-                    A::template update_multi<C>(q0, d, e, p);
-                }
-            } else if (!uniform) {
-#pragma unroll
-                for (int c = 0; c < C; c++) {
-                    for_each_material(m[c], [&](int32_t mm) {
-                        cdp q = (cdp)k.qm + (uint64_t)mm * k.qstride;
-                        if (A::has_qm(q)) {
-                            A::update(q, d[c], e[c], p[c]);
-                        }
-                    });
-                }
-            }
+        if constexpr (!RED) {
+            density_update();
         }
 
         /* record copy (solver_cpu_fdtd.tpp:288-325) */
@@ -429,11 +444,11 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
     }
 }
 
-template<class A, int C, typename ST>
+template<class A, int C, typename ST, bool RED = false>
 __global__ __launch_bounds__(64) void tile_kernel(KArgs k)
 {
-    tile_body<A, C, ST>(k, k.tile_list ? k.tile_list[blockIdx.x]
-                                       : blockIdx.x);
+    tile_body<A, C, ST, RED>(k, k.tile_list ? k.tile_list[blockIdx.x]
+                                            : blockIdx.x);
 }
 
 /* ---- launcher registry ------------------------------------------------- */
@@ -446,12 +461,13 @@ struct kernel_entry {
     uint32_t form; /* MBH_FORM_* the kernel implements */
     uint32_t cells_per_lane;
     tile_launch_fn fn;
+    uint32_t order; /* MBH_ORDER_* */
 };
 
-template<class A, int C, typename ST>
+template<class A, int C, typename ST, bool RED = false>
 hipError_t launch_tile(const KArgs& k, uint64_t num_tiles, hipStream_t stream)
 {
-    hipLaunchKernelGGL((tile_kernel<A, C, ST>), dim3((unsigned)num_tiles),
+    hipLaunchKernelGGL((tile_kernel<A, C, ST, RED>), dim3((unsigned)num_tiles),
                        dim3(64), 0, stream, k);
     return hipGetLastError();
 }
@@ -484,6 +500,8 @@ extern const kernel_entry k_table_zform[];
 extern const uint32_t k_table_zform_n;
 extern const kernel_entry k_table_zm[];
 extern const uint32_t k_table_zm_n;
+extern const kernel_entry k_table_red[];
+extern const uint32_t k_table_red_n;
 extern const zconv_entry z_table[];
 extern const uint32_t z_table_n;
 

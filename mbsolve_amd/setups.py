@@ -117,12 +117,14 @@ def _make_qm(mb, q):
     return mb.qm_description(q["carrier_density"], H, u, sop)
 
 
-def solver_name(spec, form=""):
-    """hip-fdtd-* name for the spec's algorithm ('' / 'mixed' / 'zform')."""
+def solver_name(spec, form="", order="plain"):
+    """hip-fdtd-* name for the spec's algorithm ('' / 'mixed' / 'zform' /
+    'mfma' / 'mfma64'); order 'red' gives the hip-fdtd-red-* name."""
     a = spec["algo"]
-    base = ("hip-fdtd-noop" if a == "noop" else
-            "hip-fdtd-%dlvl-%s" % (spec["num_levels"],
-                                  "rodr" if a == "cvr-rodr" else a))
+    pre = "hip-fdtd-red-" if order == "red" else "hip-fdtd-"
+    base = (pre + "noop" if a == "noop" else
+            pre + "%dlvl-%s" % (spec["num_levels"],
+                                "rodr" if a == "cvr-rodr" else a))
     return base + ("-" + form if form else "")
 
 

@@ -1,11 +1,12 @@
 """CPU FP64 ORACLE -- test infrastructure only.
 
 Restates the reference mbsolve's plain FDTD solver
-(solver-cpu/include/mbsolve/solver-cpu/solver_cpu_fdtd.tpp) so that the HIP
-path can be checked against it:
+(solver-cpu/include/mbsolve/solver-cpu/solver_cpu_fdtd.tpp) and its red
+variant (solver_cpu_fdtd_red.tpp) so that the HIP path can be checked
+against them:
 
-* ``liboracle.so`` (fdtd_oracle.c): the time loop and the per-cell density
-  updates, run on an ``mbh_problem`` (the same struct the GPU gets);
+* ``liboracle.so`` (fdtd_oracle.c): the two time loops and the per-cell
+  density updates, run on an ``mbh_problem`` (the same struct the GPU gets);
 * ``qm_setup``: an independent numpy/scipy restatement of the constructor's
   set-up (grid, coefficients, Cayley/Rodrigues constants, records, sources).
 
@@ -33,6 +34,9 @@ def load():
         lib.orc_run.restype = C.c_int
         lib.orc_run.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, C.c_uint64,
                                 C.c_uint64, C.c_int]
+        lib.orc_run_red.restype = C.c_int
+        lib.orc_run_red.argtypes = [C.c_void_p, dp, dp, dp, dp, dp,
+                                    C.c_uint64, C.c_uint64, C.c_int, C.c_int]
         lib.orc_init_state.restype = None
         lib.orc_init_state.argtypes = [C.c_void_p, dp, dp, dp, dp]
         lib.orc_cayley_update.restype = None
@@ -69,12 +73,27 @@ class OracleRun:
         self.lib.orc_init_state(C.byref(self.p.struct), _ptr(self.e),
                                 _ptr(self.h), _ptr(self.pp), _ptr(self.d))
 
-    def run(self, n_begin=0, n_end=None, threads=0):
+    def run(self, n_begin=0, n_end=None, threads=0, order="plain",
+            overlap=0):
+        """Steps [n_begin, n_end) in the plain order (solver_cpu_fdtd) or the
+        red order (solver_cpu_fdtd_red; `overlap` = its OL, 0 = the
+        reference's default)."""
         if n_end is None:
             n_end = int(self.p.dict["num_timesteps"])
-        self.lib.orc_run(C.byref(self.p.struct), _ptr(self.e), _ptr(self.h),
-                         _ptr(self.pp), _ptr(self.d), _ptr(self.scratch),
-                         n_begin, n_end, threads)
+        if order == "red":
+            rc = self.lib.orc_run_red(
+                C.byref(self.p.struct), _ptr(self.e), _ptr(self.h),
+                _ptr(self.pp), _ptr(self.d), _ptr(self.scratch), n_begin,
+                n_end, threads, overlap)
+            if rc != 0:
+                raise RuntimeError("orc_run_red failed (allocation or thread "
+                                   "team size)")
+        elif order == "plain":
+            self.lib.orc_run(C.byref(self.p.struct), _ptr(self.e),
+                             _ptr(self.h), _ptr(self.pp), _ptr(self.d),
+                             _ptr(self.scratch), n_begin, n_end, threads)
+        else:
+            raise ValueError("order must be 'plain' or 'red'")
         return self
 
     def records(self):

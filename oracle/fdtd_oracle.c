@@ -3,6 +3,8 @@
  *
  * Restates the reference's plain FDTD time loop
  *   solver-cpu/include/mbsolve/solver-cpu/solver_cpu_fdtd.tpp:223-338
+ * its red (redundant-halo, density-first) time loop
+ *   solver-cpu/include/mbsolve/solver-cpu/solver_cpu_fdtd_red.tpp:351-567
  * and its per-cell density updates
  *   lindblad_noop       mbsolve-lib/include/mbsolve/lib/internal/
  *                       algo_lindblad_noop.hpp:53-57
@@ -20,6 +22,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <omp.h>
@@ -511,6 +514,277 @@ int orc_run(const mbh_problem* P, double* e, double* h, double* p,
         }
     }
     return 0;
+}
+
+/* Red time loop of solver_cpu_fdtd_red::run
+ * (solver_cpu_fdtd_red.tpp:351-567), steps n in [n_begin, n_end).  Each of
+ * T threads owns chunk_base = N_x / T cells (the last one also the
+ * remainder) and keeps them with OL halo cells on each side in arrays of
+ * its own (:94-201; cells outside the grid hold zeros and are not updated).
+ * Every OL steps the halos of d, e and h are copied from the neighbours'
+ * arrays (:422-444); then OL sub-steps m run on a range that shrinks by one
+ * cell per side and step: density (:451-457), E (:459-461, update_e
+ * :288-307 with the border aligned down to VEC = 4), sources (:463-470,
+ * apply_sources :326-349, which skips a source at a thread's local index
+ * 0), H (:472-473, update_h :309-324), boundary h[0] = h[N_x] = 0
+ * (:476-481), records of the owned cells (:484-549).  OL defaults to the
+ * reference's 8 (1 for six levels, :45); T is lowered until every chunk
+ * holds at least OL cells, which the halo copy needs.  The global arrays
+ * hold the state before and after (owned cells gathered). */
+#define RED_VEC 4
+int orc_run_red(const mbh_problem* P, double* e, double* h, double* p,
+                double* d, double* scratch, uint64_t n_begin, uint64_t n_end,
+                int threads, int overlap)
+{
+    const int64_t nx = (int64_t)P->num_gridpoints;
+    const int N = (int)P->num_levels;
+    const int ds = density_stride(P);
+    const uint64_t qs = P->qm_stride;
+    const double dx_inv = P->dx_inv;
+    const int64_t OL = overlap > 0 ? overlap : (N == 6 ? 1 : 8);
+    int T = threads > 0 ? threads : omp_get_max_threads();
+    while (T > 1 && nx / T < OL) {
+        T--;
+    }
+    const int64_t chunk_base = nx / T;
+    const int64_t chunk_rem = nx % T;
+
+    double** te = calloc(T, sizeof(double*));
+    double** th = calloc(T, sizeof(double*));
+    double** tp = calloc(T, sizeof(double*));
+    double** td = calloc(T, sizeof(double*));
+    int32_t** tm = calloc(T, sizeof(int32_t*));
+    int ok = te && th && tp && td && tm;
+    for (int t = 0; ok && t < T; t++) {
+        const int64_t chunk = chunk_base + (t == T - 1 ? chunk_rem : 0);
+        const int64_t size = chunk + 2 * OL;
+        te[t] = calloc(size, sizeof(double));
+        th[t] = calloc(size, sizeof(double));
+        tp[t] = calloc(size, sizeof(double));
+        td[t] = calloc(size * (ds ? ds : 1), sizeof(double));
+        tm[t] = calloc(size, sizeof(int32_t));
+        ok = te[t] && th[t] && tp[t] && td[t] && tm[t];
+    }
+    int team_ok = ok;
+
+    if (ok) {
+#pragma omp parallel num_threads(T)
+        {
+            const int tid = omp_get_thread_num();
+            if (omp_get_num_threads() != T) {
+#pragma omp atomic write
+                team_ok = 0;
+            }
+#pragma omp barrier
+            if (team_ok) {
+                const int64_t chunk =
+                    chunk_base + (tid == T - 1 ? chunk_rem : 0);
+                const int64_t size = chunk + 2 * OL;
+                const int64_t base = tid * chunk_base - OL;
+                double* E = te[tid];
+                double* H = th[tid];
+                double* Pp = tp[tid];
+                double* D = td[tid];
+                int32_t* M = tm[tid];
+
+                /* load this thread's cells, halo included */
+                for (int64_t i = 0; i < size; i++) {
+                    const int64_t g = base + i;
+                    if (g >= 0 && g < nx) {
+                        M[i] = (int32_t)P->mat_idx[g];
+                        E[i] = e[g];
+                        H[i] = h[g];
+                        Pp[i] = p[g];
+                        for (int j = 0; j < ds; j++) {
+                            D[i * ds + j] = d[g * ds + j];
+                        }
+                    } else {
+                        M[i] = -1;
+                        H[i] = (g == nx) ? h[nx] : 0.0;
+                    }
+                }
+#pragma omp barrier
+
+                for (uint64_t n = n_begin; n < n_end; n += OL) {
+                    const uint64_t sub =
+                        (n_end - n < (uint64_t)OL) ? n_end - n : (uint64_t)OL;
+
+                    /* halo exchange (d, e, h) */
+                    if (tid > 0) {
+                        const double* pe = te[tid - 1];
+                        const double* ph = th[tid - 1];
+                        const double* pd = td[tid - 1];
+                        for (int64_t i = 0; i < OL; i++) {
+                            E[i] = pe[chunk_base + i];
+                            H[i] = ph[chunk_base + i];
+                            for (int j = 0; j < ds; j++) {
+                                D[i * ds + j] = pd[(chunk_base + i) * ds + j];
+                            }
+                        }
+                    }
+                    if (tid < T - 1) {
+                        const double* ne = te[tid + 1];
+                        const double* nh = th[tid + 1];
+                        const double* nd = td[tid + 1];
+                        for (int64_t i = 0; i < OL; i++) {
+                            E[OL + chunk_base + i] = ne[OL + i];
+                            H[OL + chunk_base + i] = nh[OL + i];
+                            for (int j = 0; j < ds; j++) {
+                                D[(OL + chunk_base + i) * ds + j] =
+                                    nd[(OL + i) * ds + j];
+                            }
+                        }
+                    }
+#pragma omp barrier
+
+                    for (uint64_t m = 0; m < sub; m++) {
+                        const uint64_t step = n + m;
+                        const int64_t border =
+                            (int64_t)m - (int64_t)(m % RED_VEC);
+
+                        /* density update */
+                        if (P->algo != MBH_ALGO_NOOP) {
+                            for (int64_t i = (int64_t)m;
+                                 i < size - (int64_t)m - 1; i++) {
+                                if (M[i] < 0) {
+                                    continue;
+                                }
+                                const double* q = P->qm_consts + qs * M[i];
+                                if (P->algo == MBH_ALGO_REG_CAYLEY) {
+                                    cay_update(N, q, D + i * ds, E[i], &Pp[i]);
+                                } else {
+                                    rodr_update(N, q, D + i * ds, E[i],
+                                                &Pp[i]);
+                                }
+                            }
+                        }
+                        /* E update */
+                        for (int64_t i = border; i < size - border - 1; i++) {
+                            if (M[i] < 0) {
+                                continue;
+                            }
+                            const double* f = P->fdtd_consts + 4 * M[i];
+                            E[i] = f[0] * E[i] +
+                                f[1] * (-f[3] * Pp[i] + (H[i + 1] - H[i]) *
+                                        dx_inv);
+                        }
+                        /* sources, in list order */
+                        for (uint32_t k = 0; k < P->num_sources; k++) {
+                            const int64_t at = (int64_t)P->sources[k].x_idx -
+                                tid * chunk_base + OL;
+                            if (at > 0 && at < chunk + 2 * OL) {
+                                const double v = P->source_data
+                                    [(uint64_t)k * P->num_timesteps + step];
+                                if (P->sources[k].type == MBH_SRC_HARD) {
+                                    E[at] = v;
+                                } else {
+                                    E[at] = E[at] + v;
+                                }
+                            }
+                        }
+                        /* H update and boundary condition */
+                        for (int64_t i = border + 1; i < size - border; i++) {
+                            if (M[i] < 0) {
+                                continue;
+                            }
+                            const double* f = P->fdtd_consts + 4 * M[i];
+                            H[i] = H[i] + f[2] * (E[i] - E[i - 1]);
+                        }
+                        if (tid == 0) {
+                            H[OL] = 0.0;
+                        }
+                        if (tid == T - 1) {
+                            H[OL + chunk] = 0.0;
+                        }
+                        /* records of the owned cells */
+                        for (uint32_t r = 0; r < P->num_records; r++) {
+                            const mbh_record* R = &P->records[r];
+                            if (!hasto_record(R->interval_ratio, step)) {
+                                continue;
+                            }
+                            const uint64_t row =
+                                (uint64_t)floor(R->interval_ratio * step);
+                            if (row >= R->rows) {
+                                continue;
+                            }
+                            if (R->type == MBH_REC_DENSITY &&
+                                R->row_idx != R->col_idx) {
+                                continue;
+                            }
+                            double* dst =
+                                scratch + R->offset_real + row * R->cols;
+                            const int64_t pos = (int64_t)R->position_idx;
+                            for (int64_t i = OL; i < chunk + OL; i++) {
+                                const int64_t g = base + i;
+                                if (g < pos || g >= pos + (int64_t)R->cols) {
+                                    continue;
+                                }
+                                const double* di = D + i * ds;
+                                double val = 0.0;
+                                switch (R->type) {
+                                case MBH_REC_ELECTRIC:
+                                    val = E[i];
+                                    break;
+                                case MBH_REC_POLAR_DT:
+                                    val = Pp[i];
+                                    break;
+                                case MBH_REC_MAGNETIC:
+                                    val = H[i];
+                                    break;
+                                case MBH_REC_INVERSION:
+                                    if (P->algo == MBH_ALGO_REG_CAYLEY) {
+                                        val = di[2 * (N + 1)] - di[0];
+                                    } else if (P->algo == MBH_ALGO_CVR_RODR) {
+                                        val = di[N * (N - 1)];
+                                    }
+                                    break;
+                                default:
+                                    if (P->algo == MBH_ALGO_REG_CAYLEY) {
+                                        val = di[2 * (R->row_idx * N +
+                                                      R->row_idx)];
+                                    } else if (P->algo == MBH_ALGO_CVR_RODR) {
+                                        val = cv_population(N, di,
+                                                            (int)R->row_idx);
+                                    }
+                                    break;
+                                }
+                                dst[g - pos] = val;
+                            }
+                        }
+                    }
+#pragma omp barrier
+                }
+
+                /* gather the owned cells (and h[N_x]) */
+                for (int64_t i = OL; i < chunk + OL; i++) {
+                    const int64_t g = base + i;
+                    e[g] = E[i];
+                    h[g] = H[i];
+                    p[g] = Pp[i];
+                    for (int j = 0; j < ds; j++) {
+                        d[g * ds + j] = D[i * ds + j];
+                    }
+                }
+                if (tid == T - 1) {
+                    h[nx] = H[OL + chunk];
+                }
+            }
+        }
+    }
+
+    for (int t = 0; t < T; t++) {
+        if (te) free(te[t]);
+        if (th) free(th[t]);
+        if (tp) free(tp[t]);
+        if (td) free(td[t]);
+        if (tm) free(tm[t]);
+    }
+    free(te);
+    free(th);
+    free(tp);
+    free(td);
+    free(tm);
+    return (ok && team_ok) ? 0 : -1;
 }
 
 /* single-cell helpers for unit tests */

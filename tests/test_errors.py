@@ -34,8 +34,14 @@ def test_names_registered_once():
     for n in range(3, 7):
         assert "hip-fdtd-%dlvl-rodr-mfma" % n in names
         assert "hip-fdtd-%dlvl-rodr-mfma64" % n in names
+    assert "hip-fdtd-red-noop" in names
+    for n in range(2, 7):
+        assert "hip-fdtd-red-%dlvl-rodr" % n in names
+        assert "hip-fdtd-red-%dlvl-reg-cayley" % n in names
     for bad in ("hip-fdtd-2lvl-rodr-mfma", "hip-fdtd-2lvl-reg-cayley-mfma",
-                "hip-fdtd-3lvl-reg-cayley-zform", "hip-fdtd-noop-zform"):
+                "hip-fdtd-3lvl-reg-cayley-zform", "hip-fdtd-noop-zform",
+                "hip-fdtd-red-3lvl-rodr-mixed", "hip-fdtd-red-2lvl-rodr-zform",
+                "hip-fdtd-red-noop-mixed"):
         assert bad not in names
     mb.load_hip_solvers()  # a second loader is harmless
     assert mb.solver.get_avail_solvers() == names
@@ -165,6 +171,29 @@ def test_capi_rejects_unsupported_forms(form, code, msg):
     with pytest.raises(capi.MBHError, match=msg) as ei:
         capi.Context([d], capi.options(form))
     assert ei.value.code == code
+
+
+@pytest.mark.parametrize("form,order,code,msg", [
+    ("mixed", "red", 4, "red step order"),   # red order: FP64 form only
+    ("zform64", "red", 4, "red step order"),
+    ("fp64", 2, 1, "unknown step order"),
+])
+def test_capi_rejects_unsupported_orders(form, order, code, msg):
+    d = _good_problem()
+    with pytest.raises(capi.MBHError, match=msg) as ei:
+        capi.Context([d], capi.options(form, order=order))
+    assert ei.value.code == code
+
+
+def test_red_names_parse_without_gpu():
+    spec = setups.v3_rodr(num_gridpoints=64, endtime=2e-14, num_records=4)
+    dev, scen = setups.build(spec)
+    red = mb.build_problem(setups.solver_name(spec, order="red"), dev, scen)
+    plain = mb.build_problem(setups.solver_name(spec), dev, scen)
+    assert setups.solver_name(spec, order="red") == "hip-fdtd-red-3lvl-rodr"
+    assert np.array_equal(red["qm_consts"], plain["qm_consts"])
+    with pytest.raises(ValueError, match="Unknown solver"):
+        mb.build_problem("hip-fdtd-red-3lvl-rodr-mixed", dev, scen)
 
 
 def test_capi_mfma_needs_three_levels():

@@ -17,9 +17,9 @@ FP64_TOL = 1e-12   # bound for the FP64 form against the oracle
 MIXED_TOL = 1e-5   # bound for the FP32-storage form
 
 
-def _run_hip(spec, form="", **opts):
+def _run_hip(spec, form="", order="plain", **opts):
     dev, scen = setups.build(spec)
-    name = setups.solver_name(spec, form)
+    name = setups.solver_name(spec, form, order)
     print("case", spec["name"], name, opts, flush=True)
     sol = mb.create_hip_solver(name, dev, scen, **opts)
     sol.run()
@@ -189,3 +189,35 @@ def test_mfma_batch_mixed_tiles(gpu):
             err = (float(np.max(np.abs(a - b)))
                    if x.get_name().startswith("inv") else rel_err(a, b))
             assert err <= MFMA_TOL, (r, x.get_name(), err)
+
+
+RED = SMALL[:3] + [
+    ("noop", lambda: setups.maxwell_line(num_gridpoints=20000,
+                                         num_timesteps=600)),
+    ("cay5", lambda: dict(setups.marskar2011(5, num_gridpoints=400,
+                                             endtime=0.08e-12),
+                          algo="reg-cayley")),
+]
+
+
+@pytest.mark.parametrize("case", [c[0] for c in RED])
+def test_red_order_matches_red_oracle(gpu, case):
+    """hip-fdtd-red-* (density first, solver_cpu_fdtd_red) against the
+    oracle's red loop (threads and overlap as the reference's defaults)."""
+    spec = dict(RED)[case]()
+    sol, recs = _run_hip(spec, order="red")
+    assert sol.info()["step_order"] == capi.MBH_ORDER_RED
+    ref = oracle.OracleRun(sol.problem()).run(order="red").records()
+    for a, b in zip(recs, ref):
+        assert a.shape == b.shape
+        assert rel_err(a, b) <= FP64_TOL, (case, rel_err(a, b))
+
+
+@pytest.mark.parametrize("tb,tile", [(1, 64), (7, 128), (24, 128)])
+def test_red_order_independent_of_tiling(gpu, tb, tile):
+    spec = setups.sit_2lvl(num_gridpoints=1500, endtime=0.5e-12,
+                           num_records=16)
+    _, base = _run_hip(spec, order="red", temporal_block=1, tile_cells=64)
+    _, recs = _run_hip(spec, order="red", temporal_block=tb, tile_cells=tile)
+    for a, b in zip(recs, base):
+        assert np.array_equal(a, b)
