@@ -156,26 +156,16 @@ def _cpu_model():
     return "unknown CPU"
 
 
-def cpu_baseline(specs, form_name, seconds=15.0, reps=3):
-    """Bounded sample of the CPU FP64 oracle on the same workload: the full
-    grid of run 0, `reps` timed pieces of as many steps as fit in about
-    `seconds` together; the median rate is reported.  OpenMP threads are
-    pinned to cores (OMP_PROC_BIND=close, OMP_PLACES=cores, set in main
-    before the oracle library loads)."""
-    import mbsolve_amd.lib as mb
-    from mbsolve_amd import setups
+def _cpu_rate(prob, order, seconds, reps, threads):
+    """Median cell-update rate of `reps` timed pieces of the oracle's
+    `order` loop, as many steps as fit in about `seconds` together."""
     import oracle
-    spec = specs[0]
-    dev, scen = setups.build(spec)
-    prob = mb.build_problem(setups.solver_name(spec), dev, scen)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
-        min(16, os.cpu_count() or 1)
     run = oracle.OracleRun(prob)
     nx = int(prob["num_gridpoints"])
     probe = 10
-    run.run(0, probe, threads=threads)  # warm-up (thread start, caches)
+    run.run(0, probe, threads=threads, order=order)  # warm-up
     t0 = time.perf_counter()
-    run.run(probe, 2 * probe, threads=threads)
+    run.run(probe, 2 * probe, threads=threads, order=order)
     dt = time.perf_counter() - t0
     avail = int(prob["num_timesteps"]) - 2 * probe
     per = int(max(1, min(avail // reps,
@@ -184,20 +174,46 @@ def cpu_baseline(specs, form_name, seconds=15.0, reps=3):
     n = 2 * probe
     for _ in range(reps):
         t0 = time.perf_counter()
-        run.run(n, n + per, threads=threads)
+        run.run(n, n + per, threads=threads, order=order)
         el = time.perf_counter() - t0
         rates.append(nx * per / el)
         total += el
         n += per
     rates.sort()
-    return {"value": rates[len(rates) // 2], "unit": "cell-updates/s",
+    return rates[len(rates) // 2], per, total, 2 * probe
+
+
+def cpu_baseline(specs, form_name, seconds=15.0, reps=3):
+    """Bounded sample of the CPU FP64 oracle on the same workload: the full
+    grid of run 0, both CPU time loops of the reference (plain
+    solver_cpu_fdtd and the redundant-halo solver_cpu_fdtd_red), each for
+    `reps` timed pieces of as many steps as fit in about `seconds` / 2
+    together; the faster loop's median rate is reported.  OpenMP threads are
+    pinned to cores (OMP_PROC_BIND=close, OMP_PLACES=cores, set in main
+    before the oracle library loads)."""
+    import mbsolve_amd.lib as mb
+    from mbsolve_amd import setups
+    spec = specs[0]
+    dev, scen = setups.build(spec)
+    prob = mb.build_problem(setups.solver_name(spec), dev, scen)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
+        min(16, os.cpu_count() or 1)
+    nx = int(prob["num_gridpoints"])
+    res = {o: _cpu_rate(prob, o, seconds / 2, reps, threads)
+           for o in ("plain", "red")}
+    best = max(res, key=lambda o: res[o][0])
+    rate, per, total, warm = res[best]
+    return {"value": rate, "unit": "cell-updates/s",
             "cores": threads, "kind": "port",
-            "sample": "oracle (restated cpu-fdtd plain solver, FP64) on the "
-                      "full %d-cell grid, median of %d pieces of %d steps "
-                      "after %d warm-up steps, %.1f s in total, %d OpenMP "
-                      "threads pinned to cores of %s"
-                      % (nx, reps, per, 2 * probe, total, threads,
-                         _cpu_model())}
+            "orders": {o: r[0] for o, r in res.items()},
+            "sample": "oracle (restated cpu-fdtd %s solver, FP64; the "
+                      "faster of the plain and red loops, plain %.3g, red "
+                      "%.3g cu/s) on the full %d-cell grid, median of %d "
+                      "pieces of %d steps after %d warm-up steps, %.1f s, "
+                      "%d OpenMP threads pinned to cores of %s"
+                      % ("red" if best == "red" else "plain",
+                         res["plain"][0], res["red"][0], nx, reps, per, warm,
+                         total, threads, _cpu_model())}
 
 
 def measure(specs, form, warmup, steps, opts):
