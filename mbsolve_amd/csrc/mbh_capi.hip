@@ -182,6 +182,24 @@ void bloch2_from_sigma(const double* o, const double* sig, double* rho)
     rho[3] = -g * sig[2];
 }
 
+/* y* = -Q^T d_in of a z-form blob (layout algo_zform<N>): the state of
+ * the relaxation equilibrium d = -d_in, the fixed point of the step at
+ * zero field */
+void zform_yeq(uint32_t levels, const double* zq, double* yeq)
+{
+    const int K = (int)(levels * levels) - 1;
+    const int B = K / 2;
+    const double* Qt = zq + 3 + B + 2 * K * K + 3 * K; /* OFF_QT */
+    const double* din = Qt + 2 * K * K;                /* OFF_DIN */
+    for (int i = 0; i < K; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < K; j++) {
+            acc += Qt[i * K + j] * din[j];
+        }
+        yeq[i] = (zq[0] != 0.0) ? -acc : 0.0;
+    }
+}
+
 /* MFMA-form constants of one material (layout zm_geom<N>) from its
  * z-form blob (layout algo_zform<N>): split FP16 fragments of the scaled
  * W - I in the kernel's row order, and per-lane rows of theta, a, c and
@@ -197,6 +215,9 @@ void zm_build(const double* zq, unsigned char* out)
     if (zq[0] == 0.0) {
         return;
     }
+    static_assert(Z::OFF_QT == 3 + Gm::B + 2 * K * K + 3 * K &&
+                      Z::OFF_DIN == Z::OFF_QT + 2 * K * K,
+                  "z-form blob layout (zform_yeq)");
     const double* W = zq + Z::OFF_W;
     const double* cv = zq + Z::OFF_C;
     double dmax = 0.0;
@@ -618,6 +639,15 @@ uint32_t kernel_form(uint32_t algo, uint32_t levels, uint32_t form)
     return form;
 }
 
+/* The VALU z-form with FP32 storage keeps y - y* in memory (y* appended to
+ * the device blobs), so the storage rounding scales with the distance from
+ * the relaxation equilibrium; the MFMA forms keep y. */
+uint32_t yshift(const mbh_ctx* ctx)
+{
+    return (ctx->zform && ctx->fp32 && ctx->form == MBH_FORM_MIXED) ? 1u
+                                                                    : 0u;
+}
+
 /* Cells owned by tiles: the trailing pseudo cell is never written. */
 uint64_t owned_cells(const mbh_ctx* ctx) { return ctx->G - 1; }
 
@@ -858,6 +888,7 @@ KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
     k.fdtd = ctx->fdtd;
     k.qm = ctx->qm;
     k.qstride = ctx->qstride;
+    k.yshift = yshift(ctx);
     k.src = ctx->src;
     k.tile_src = ctx->tile_src;
     k.rec = ctx->rec;
@@ -1041,9 +1072,11 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     std::vector<int32_t> mat(G, -1);
     std::vector<uint8_t> hz(G, 1);
     std::vector<double> fdtd((size_t)nmat_total * FDTD_STRIDE);
+    /* the device copy of a z-form blob gets y* appended (OFF_YEQ) */
+    const uint64_t src_stride =
+        c->zform ? runs[0].zform_stride : runs[0].qm_stride;
     c->qstride = c->bloch ? BLOCH2_STRIDE
-                          : (c->zform ? runs[0].zform_stride
-                                      : runs[0].qm_stride);
+                          : (c->zform ? src_stride + c->dref : src_stride);
     std::vector<double> qm((size_t)nmat_total * c->qstride);
     uint32_t moff = 0;
     for (uint32_t r = 0; r < num_runs; r++) {
@@ -1064,9 +1097,12 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
                 bloch2_constants(q + (size_t)m * p.qm_stride,
                                  &qm[(size_t)(moff + m) * c->qstride]);
             } else if (c->qstride) {
-                std::memcpy(&qm[(size_t)(moff + m) * c->qstride],
-                            q + (size_t)m * c->qstride,
-                            c->qstride * sizeof(double));
+                double* dst = &qm[(size_t)(moff + m) * c->qstride];
+                std::memcpy(dst, q + (size_t)m * src_stride,
+                            src_stride * sizeof(double));
+                if (c->zform) {
+                    zform_yeq(c->levels, dst, dst + src_stride);
+                }
             }
         }
         moff += p.num_materials;
@@ -1198,7 +1234,7 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
                 double* dtmp = nullptr;
                 RC(upload(c, &dtmp, d.data(), d.size()));
                 ZArgs za{ dtmp, c->state[0][3], G, c->dstride, c->mat, c->qm,
-                          c->qstride };
+                          c->qstride, yshift(c) };
                 HIP_TRY(c, c->to_y(za, c->stream));
                 HIP_TRY(c, hipStreamSynchronize(c->stream));
             } else if (c->fp32) {
@@ -1461,7 +1497,7 @@ int mbh_fetch_state(mbh_ctx* ctx, uint32_t run, double* e, double* h,
                 return rc;
             }
             ZArgs za{ dtmp, S[3], ctx->G, ctx->dstride, ctx->mat, ctx->qm,
-                      ctx->qstride };
+                      ctx->qstride, yshift(ctx) };
             HIP_TRY(ctx, ctx->from_y(za, ctx->stream));
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
             for (uint32_t j = 0; j < ctx->dref; j++) {

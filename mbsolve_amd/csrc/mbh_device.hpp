@@ -167,6 +167,7 @@ __device__ __forceinline__ void cinverse(const cplx* A, cplx* R)
  * ==================================================================== */
 struct algo_noop {
     static constexpr int STAGE = 0;
+    static constexpr bool SHIFT = false;
     static constexpr bool FAST_FDTD = false;
     static constexpr int DC = 0; /* stored density components */
     static constexpr int STATE = 1;
@@ -196,6 +197,7 @@ struct algo_noop {
 template<int N>
 struct algo_cayley {
     static constexpr int STAGE = 0;
+    static constexpr bool SHIFT = false;
     static constexpr bool FAST_FDTD = false;
     static constexpr int DC = 2 * N * N;
     static constexpr int STATE = 2 * N * N;
@@ -366,6 +368,7 @@ struct algo_cayley {
  * ==================================================================== */
 struct algo_bloch2 {
     static constexpr int STAGE = 0;
+    static constexpr bool SHIFT = false;
     static constexpr int DC = 4;
     static constexpr int STATE = 4;
     static constexpr bool FAST_FDTD = true;
@@ -495,6 +498,7 @@ __device__ __forceinline__ double cv_population(const double* d, int m)
 template<int N>
 struct algo_rodr {
     static constexpr int STAGE = 0;
+    static constexpr bool SHIFT = false;
     static constexpr bool FAST_FDTD = false;
     static constexpr int K = N * N - 1;
     static constexpr int B = K / 2;
@@ -733,6 +737,13 @@ struct algo_zform {
     static constexpr int OFF_QT = OFF_O + K;
     static constexpr int OFF_QTA1 = OFF_QT + K * K;
     static constexpr int OFF_DIN = OFF_QTA1 + K * K;
+    /* appended to the device copy of the blob (not part of the C ABI
+     * layout): y* = -Qt d_in, the fixed point of the step at zero field
+     * (d = -d_in, the relaxation equilibrium).  FP32 storage holds
+     * y - y*, so its rounding scales with the distance from equilibrium
+     * instead of with |y*|. */
+    static constexpr int OFF_YEQ = OFF_DIN + K;
+    static constexpr bool SHIFT = true;
 
     __device__ static bool has_qm(cdp q) { return q[0] != 0.0; }
 

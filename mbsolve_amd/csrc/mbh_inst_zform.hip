@@ -2,7 +2,8 @@
  * Kernel instantiations: Rodrigues in the rotated basis (z-form) with FP32
  * (MBH_FORM_MIXED) or FP64 (MBH_FORM_ZFORM64) state storage, plus the
  * conversions between the reference coherence vector d and the z-form
- * state y:  y = Q^T A1 (d + d_in) - Q^T d_in,   d = P y + o.
+ * state y:  y = Q^T A1 (d + d_in) - Q^T d_in,   d = P y + o
+ * (FP32 storage of the VALU z-form holds y - y*, y* = -Q^T d_in).
  */
 #include "mbh_tile.hpp"
 
@@ -42,6 +43,13 @@ __global__ __launch_bounds__(64) void zform_to_y(ZArgs a)
                 }
                 y[i] = acc;
             }
+            if (std::is_same<ST, float>::value && a.yshift) {
+                /* FP32 storage holds y - y* */
+#pragma unroll
+                for (int i = 0; i < K; i++) {
+                    y[i] -= q[algo_zform<N>::OFF_YEQ + i];
+                }
+            }
         }
     });
     if (in) {
@@ -70,6 +78,12 @@ __global__ __launch_bounds__(64) void zform_from_y(ZArgs a)
     for_each_material(m, [&](int32_t mm) {
         cdp q = (cdp)a.zq + (uint64_t)mm * a.zstride;
         if (q[0] != 0.0) {
+            if (std::is_same<ST, float>::value && a.yshift) {
+#pragma unroll
+                for (int j = 0; j < K; j++) {
+                    s.v[j] += q[algo_zform<N>::OFF_YEQ + j];
+                }
+            }
 #pragma unroll
             for (int j = 0; j < K; j++) {
                 d[j] = algo_zform<N>::dcomp(q, s, j);

@@ -27,6 +27,8 @@
 #ifndef MBH_TILE_HPP
 #define MBH_TILE_HPP
 
+#include <type_traits>
+
 #include "mbh_device.hpp"
 
 namespace mbh {
@@ -81,7 +83,7 @@ struct KArgs {
     uint32_t nsteps;
     uint32_t halo;
     uint32_t owned;
-    uint32_t pad;
+    uint32_t yshift; /* FP32 z-form storage holds y - y* (VALU z-form) */
     const void* frag;     /* MFMA form: [nmat][frag_stride bytes] */
     uint64_t frag_stride;
     const uint32_t* tile_list; /* tiles of this launch (NULL: blockIdx) */
@@ -198,6 +200,25 @@ __device__ __forceinline__ double record_value(const KArgs& k, uint32_t type,
     }
 }
 
+/* FP32 storage of the z-form holds y - y* (y* at OFF_YEQ of the material's
+ * blob): v += sign * y* for a cell of material m with a density */
+template<class A>
+__device__ __forceinline__ void add_shift(const KArgs& k, int32_t m,
+                                          double* v, double sign)
+{
+    if (m < 0) {
+        return;
+    }
+    const double* q = (const double*)k.qm + (uint64_t)m * k.qstride;
+    if (q[0] == 0.0) {
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < A::DC; j++) {
+        v[j] = fma(sign, q[A::OFF_YEQ + j], v[j]);
+    }
+}
+
 /* sources staged per launch in LDS (values for the block's steps) */
 constexpr int TILE_SOURCES = 8;
 constexpr int MAX_BLOCK = 64; /* largest temporal block */
@@ -237,6 +258,11 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 #pragma unroll
         for (int j = 0; j < A::DC; j++) {
             d[c].v[j] = in ? ld<ST>(k.d_in, (uint64_t)j * k.dstride + g) : 0.0;
+        }
+        if constexpr (A::SHIFT && std::is_same<ST, float>::value) {
+            if (k.yshift) {
+                add_shift<A>(k, m[c], d[c].v, 1.0);
+            }
         }
         if (m[c] >= 0) {
             const double* f = k.fdtd + (uint64_t)m[c] * FDTD_STRIDE;
@@ -430,6 +456,11 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const int64_t g = gl + c;
+        if constexpr (A::SHIFT && std::is_same<ST, float>::value) {
+            if (k.yshift) {
+                add_shift<A>(k, m[c], d[c].v, -1.0);
+            }
+        }
         if (g >= own_lo && g < own_hi) {
             st<ST>(k.e_out, g, e[c]);
             st<ST>(k.h_out, g, h[c]);
@@ -481,6 +512,7 @@ struct ZArgs {
     const int32_t* mat;
     const double* zq;
     uint64_t zstride;
+    uint32_t yshift; /* FP32 storage holds y - y* */
 };
 typedef hipError_t (*zconv_fn)(const ZArgs& a, hipStream_t stream);
 

@@ -62,15 +62,23 @@ def build(spec, clear_library=True):
                           m.get("rel_permeability", 1.0))
         mb.material.add_to_library(mat)
         mats[mid] = mat
-    dev = mb.device(spec.get("name", "device"))
+    refl = spec.get("reflectivity")
+    dev = (mb.device(spec.get("name", "device"),
+                     mb.bc_field_reflectivity(*refl)) if refl else
+           mb.device(spec.get("name", "device")))
     for name, mid, x0, x1 in spec["regions"]:
         dev.add_region(mb.region(name, mats[mid], x0, x1))
     diag, off = spec["rho_init"]
+    # "e_init_random": seed of the reference's default random initial field
+    # (ic_field_random: N(0, 1) x 1e-15), seeded for reproducibility
+    e_ic = (mb.ic_field_random(seed=int(spec["e_init_random"]))
+            if "e_init_random" in spec else
+            mb.ic_field_const(spec.get("e_init", 0.0)))
     scen = mb.scenario(spec.get("name", "scenario"), spec["num_gridpoints"],
                        spec["endtime"],
                        mb.ic_density_const(mb.qm_operator(list(diag),
                                                           list(off))),
-                       mb.ic_field_const(spec.get("e_init", 0.0)),
+                       e_ic,
                        mb.ic_field_const(spec.get("h_init", 0.0)),
                        spec.get("num_timesteps", 2))
     for s in spec.get("sources", []):
@@ -444,6 +452,45 @@ def marskar2011(num_levels=6, num_gridpoints=8192, endtime=2e-12):
                      for i in range(1, N + 1)] +
                     [{"name": "e", "type": 0, "row": 0, "col": 0,
                       "interval": 0.0, "position": 0.0}]),
+    }
+
+
+def tzenov2016(num_gridpoints=8192, endtime=0.2e-9, seed=1):
+    """mbsolve-tool.cpp:310-392 (five-level quantum cascade laser frequency
+    comb, 5 mm active region, semi-transparent mirrors 0.8 / 0.8 that no
+    solver evaluates, no source: the lasing starts from the random initial
+    field, here seeded; E at x = 0 and the five populations at x = 0.1 mm
+    recorded every step)."""
+    e = E0
+    qm = {"num_levels": 5, "carrier_density": 5.6e21,
+          "h_diag": [0.10103 * e, 0.09677 * e, 0.09720 * e, 0.08129 * e,
+                     0.07633 * e],
+          "h_off": [0.0, 1.2329e-3 * e, -1.3447e-3 * e] + [0.0] * 7,
+          "mu_diag": [0.0] * 5,
+          "mu_off": [0.0] * 5 + [-e * 4e-9] + [0.0] * 4,
+          "scattering": [
+              [0.0, 0.4947e12, 0.0974e12, 0.8116e12, 1.0410e12],
+              [0.8245e12, 0.0, 0.1358e12, 0.6621e12, 1.1240e12],
+              [0.0229e12, 0.0469e12, 0.0, 0.0794e12, 0.0357e12],
+              [0.0047e12, 0.0029e12, 0.1252e12, 0.0, 0.2810e12],
+              [0.0049e12, 0.0049e12, 0.1101e12, 0.4949e12, 0.0]],
+          "pure_dephasing": [0.0, 1.0 / 1.6e-12, 1.0 / 1.6e-12] +
+                            [1.0 / 4.8e-12] * 7}
+    return {
+        "name": "tzenov2016", "algo": "cvr-rodr", "num_levels": 5,
+        "materials": {"AR": {"qm": qm, "rel_permittivity": 19.35,
+                             "overlap_factor": 0.9, "losses": 1300.0}},
+        "regions": [("Active region", "AR", 0.0, 5e-3)],
+        "reflectivity": (0.8, 0.8),
+        "num_gridpoints": num_gridpoints, "endtime": endtime,
+        "rho_init": ([0.0, 0.0, 1.0, 0.0, 0.0], []),
+        "e_init_random": seed,
+        "sources": [],
+        "records": ([{"name": "e0", "type": 0, "row": 1, "col": 1,
+                      "interval": 0.0, "position": 0.0}] +
+                    [{"name": "d%d%d" % (i, i), "type": 3, "row": i,
+                      "col": i, "interval": 0.0, "position": 0.1e-3}
+                     for i in range(1, 6)]),
     }
 
 

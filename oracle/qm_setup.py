@@ -383,6 +383,50 @@ def source_value(src, t):
 
 
 # ------------------------------------------------------------ problem -----
+def random_field(seed, n, mean=0.0, stddev=1.0, amplitude=1e-15):
+    """ic_field_random (mbsolve-lib/include/mbsolve/lib/scenario.hpp:
+    125-157) with a fixed seed, evaluated for cells 0..n-1 in order:
+    std::mt19937(seed) (numpy's MT19937 with the classic init_genrand
+    seeding), std::normal_distribution<double> as libstdc++ implements it
+    (generate_canonical from two 32-bit draws, Marsaglia's polar method,
+    the second value of each pair cached and returned next), times the
+    amplitude."""
+    bg = np.random.MT19937()
+    bg._legacy_seeding(seed)
+    words = iter(())
+
+    def u32():
+        nonlocal words
+        while True:
+            for w in words:
+                return int(w)
+            words = iter(bg.random_raw(1024))
+
+    def canonical():
+        lo = u32()
+        hi = u32()
+        r = (lo + hi * 4294967296.0) / 18446744073709551616.0
+        return r if r < 1.0 else math.nextafter(1.0, 0.0)
+
+    out = np.empty(n)
+    saved = None
+    for i in range(n):
+        if saved is not None:
+            v, saved = saved, None
+        else:
+            while True:
+                x = 2.0 * canonical() - 1.0
+                y = 2.0 * canonical() - 1.0
+                r2 = x * x + y * y
+                if 0.0 < r2 <= 1.0:
+                    break
+            mult = math.sqrt(-2.0 * math.log(r2) / r2)
+            saved = x * mult
+            v = y * mult
+        out[i] = (v * stddev + mean) * amplitude
+    return out
+
+
 def build_problem(spec):
     """Full host set-up of one run from a spec dict; the output matches the
     dict of mbsolve_amd.lib.build_problem."""
@@ -432,7 +476,10 @@ def build_problem(spec):
         mat_idx[i] = idx
         if ds and has_qm:
             d_init[i * ds:(i + 1) * ds] = d_qm
-    e_init = np.full(nx, float(spec.get("e_init", 0.0)))
+    if "e_init_random" in spec:
+        e_init = random_field(int(spec["e_init_random"]), nx)
+    else:
+        e_init = np.full(nx, float(spec.get("e_init", 0.0)))
     h_init = np.full(nx + 1, float(spec.get("h_init", 0.0)))
     h_init[nx] = 0.0
     records, scratch = [], 0

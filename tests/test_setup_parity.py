@@ -23,6 +23,9 @@ SPECS = {
                                            endtime=0.05e-12),
     "cpml": lambda: setups.tzenov2018_cpml(num_gridpoints=700,
                                            endtime=3e-12),
+    # five levels, random (seeded) initial field, no source
+    "tzenov": lambda: setups.tzenov2016(num_gridpoints=800, endtime=5e-12,
+                                        seed=3),
 }
 
 
@@ -92,7 +95,7 @@ def test_cayley_constants(name):
         assert np.allclose(D1.sum(axis=0), 1.0, atol=1e-15)
 
 
-@pytest.mark.parametrize("name", ["v3", "qcl6", "song", "marskar4"])
+@pytest.mark.parametrize("name", ["v3", "qcl6", "song", "marskar4", "tzenov"])
 def test_rodrigues_propagator(name):
     """A2(e) assembled from either blob equals expm(e dt U); A1 equals
     expm(M dt/2); M d_in = d_eq."""
@@ -129,7 +132,7 @@ def test_rodrigues_propagator(name):
                 assert np.abs(a2 - ref).max() < 1e-12, (name, e)
 
 
-@pytest.mark.parametrize("name", ["v3", "qcl6"])
+@pytest.mark.parametrize("name", ["v3", "qcl6", "tzenov"])
 def test_zform_blob_identities(name):
     """The rotated-basis blob reproduces the reference Strang step: from a
     random d, one reference step (A1, A2, A1) equals y -> W R y + c read out
@@ -162,7 +165,9 @@ def test_zform_blob_identities(name):
     QtA1 = z[3 + B + 3 * KK + 3 * K:3 + B + 4 * KK + 3 * K].reshape(K, K)
     rng = np.random.default_rng(1)
     d = rng.standard_normal(K) * 0.3
-    e = 2.3e8
+    # a field that turns the fastest rotation by at most 0.5 rad per step
+    # (at tens of radians the reference expm's own rounding dominates)
+    e = min(2.3e8, 0.5 / np.abs(th).max())
     # reference step
     d1 = A1 @ (d + d_in) - d_in
     d2 = scipy.linalg.expm(e * dt * U) @ d1
@@ -180,3 +185,10 @@ def test_zform_blob_identities(name):
     y2 = W @ r + c
     assert _rel(P @ y2 + oo, dref) < 1e-12
     assert abs(p - pref) <= 1e-12 * abs(pref) + 1e-300
+    # y* = -Q^T d_in is the fixed point at zero field, so the step can be
+    # written y' = r + (W - I)(r - y*) (the MFMA kernels' form) and FP32
+    # storage can hold y - y*
+    yeq = -Qt @ d_in
+    assert _rel(W @ yeq + c, yeq) < 1e-13
+    assert _rel(r + (W - np.eye(K)) @ (r - yeq), y2) < 1e-12
+    assert _rel(P @ yeq + oo, -d_in) < 1e-12
