@@ -846,26 +846,47 @@ struct algo_zform {
         for (int c = 0; c < C; c++) {
             p[c] = pa[c] + pb[c];
         }
-        /* y' = W r + c */
+        /* y' = W r + c.  With one cell per lane and a small blob (K <= 8,
+         * read from LDS) the rows are accumulated side by side (k outer):
+         * K independent FMA chains instead of one chain per row after
+         * another, at the price of holding W in VGPRs during the product.
+         * Several cells per lane already interleave C chains, and larger K
+         * keeps one row at a time (register pressure). */
+        constexpr int RG = (K <= 8 && C == 1) ? K : 1;
 #pragma unroll
-        for (int i = 0; i < K; i++) {
-            double acc[C];
-            const double ci = cv[i];
+        for (int i0 = 0; i0 < K; i0 += RG) {
+            double acc[C][RG];
 #pragma unroll
-            for (int c = 0; c < C; c++) {
-                acc[c] = ci;
-            }
+            for (int g = 0; g < RG; g++) {
+                if (i0 + g < K) {
+                    const double ci = cv[i0 + g];
 #pragma unroll
-            for (int k = 0; k < K; k++) {
-                const double w = W[i * K + k];
-#pragma unroll
-                for (int c = 0; c < C; c++) {
-                    acc[c] = fma(w, r[c][k], acc[c]);
+                    for (int c = 0; c < C; c++) {
+                        acc[c][g] = ci;
+                    }
                 }
             }
 #pragma unroll
-            for (int c = 0; c < C; c++) {
-                s[c].v[i] = acc[c];
+            for (int k = 0; k < K; k++) {
+#pragma unroll
+                for (int g = 0; g < RG; g++) {
+                    if (i0 + g < K) {
+                        const double w = W[(i0 + g) * K + k];
+#pragma unroll
+                        for (int c = 0; c < C; c++) {
+                            acc[c][g] = fma(w, r[c][k], acc[c][g]);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < RG; g++) {
+                if (i0 + g < K) {
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        s[c].v[i0 + g] = acc[c][g];
+                    }
+                }
             }
         }
     }
