@@ -183,7 +183,7 @@ def _cpu_rate(prob, order, seconds, reps, threads):
     return rates[len(rates) // 2], per, total, 2 * probe
 
 
-def cpu_baseline(specs, form_name, seconds=15.0, reps=3):
+def cpu_baseline(specs, form_name, seconds=15.0, reps=5):
     """Bounded sample of the CPU FP64 oracle on the same workload: the full
     grid of run 0, both CPU time loops of the reference (plain
     solver_cpu_fdtd and the redundant-halo solver_cpu_fdtd_red), each for
@@ -423,6 +423,10 @@ def main():
         if (t.get("temporal_block") == info["temporal_block"] and
                 t.get("tile_cells") == info["tile_cells"]):
             out["roofline"]["traffic"] = t.get("bytes_per_launch")
+    # the figure mbsolve-tool prints (mbsolve-tool.cpp:693-695): N_x T / dt
+    # per run time, i.e. n_t - 1 counted steps per n_t run steps, here for
+    # a full run at the measured per-step rate
+    out["tool_rate"] = r["value"] * (r["n_t"] - 1) / r["n_t"]
     if fp64 is not None:
         out["fp64_value"] = fp64["value"]
         out["fp64_steps"] = fp64["steps"]
