@@ -220,7 +220,8 @@ def test_hdf5_writer_after_run(gpu, tmp_path):
 
 
 @pytest.mark.parametrize("name", ["hip-fdtd-2lvl-reg-cayley",
-                                  "hip-fdtd-3lvl-rodr-zform"])
+                                  "hip-fdtd-3lvl-rodr-zform",
+                                  "hip-fdtd-red-3lvl-rodr"])
 def test_graph_replays_match_direct_launches(gpu, name):
     """Blocks of launches captured once in a hipGraph and replayed many
     times, with fetches (synchronisations, host work) between the runs:
@@ -248,3 +249,38 @@ def test_graph_replays_match_direct_launches(gpu, name):
             assert np.array_equal(s0[key], s1[key]), (a, b, key)
     for r0, r1 in zip(sols[0].get_results(), sols[1].get_results()):
         assert np.array_equal(r0.get_data_real(), r1.get_data_real())
+
+
+def test_red_order_batch_and_partial_runs(gpu):
+    """The red step order in a batch of runs equals the runs done one by
+    one, and run_steps in pieces equals one call; the final state is the
+    oracle's red loop (three-level Rodrigues, FP64 form)."""
+    dx = 150e-6 / 699  # one grid spacing, so all runs share dt and n_t
+    specs = [setups.v3_rodr(num_gridpoints=n, endtime=0.2e-12, seed=s,
+                            num_records=6, length=(n - 1) * dx,
+                            name="v%d" % s)
+             for s, n in ((1, 700), (2, 1000), (3, 450))]
+    singles = []
+    for sp in specs:
+        dev, scen = setups.build(sp)
+        name = setups.solver_name(sp, order="red")
+        print("case red single", sp["name"], flush=True)
+        sol = mb.create_hip_solver(name, dev, scen, temporal_block=5)
+        nt = sol.problem()["num_timesteps"]
+        for a, b in ((0, 7), (7, nt // 2), (nt // 2, nt)):
+            sol.run_steps(a, b)
+        sol.fetch_results()
+        st = sol.fetch_state()
+        ref = oracle.OracleRun(sol.problem()).run(order="red")
+        for k, v in ref.state().items():
+            assert rel_err(st[k], v) <= 1e-12, (sp["name"], k)
+        singles.append([r.get_data_real() for r in sol.get_results()])
+    mb.material.clear_library()
+    runs = [setups.build(sp, clear_library=False) for sp in specs]
+    print("case red batch", flush=True)
+    batch = mb.solver_hip_batch("hip-fdtd-red-3lvl-rodr", runs)
+    assert batch.info()["step_order"] == capi.MBH_ORDER_RED
+    batch.run()
+    for r, ref in enumerate(singles):
+        for x, b in zip(batch.get_results(r), ref):
+            assert np.array_equal(x.get_data_real(), b), r
