@@ -52,15 +52,16 @@ ALGO_FLOPS = {"v3": 168, "sit2": 120, "qcl6": 2622, "sweep2": 120,
               "sweep6": 2622, "maxwell": 10}
 # per workload: default candidate forms, reference form for the errors,
 # default steps of the plain FP64 timing (0: same as the candidates)
-DEFAULTS = {"v3": ("mixed,zform", "", 0),
+DEFAULTS = {"v3": ("mixed,zform,mfma,mfma64", "", 0),
             "sit2": ("mixed,zform", "", 0),
             "sweep2": ("mixed,zform", "", 0),
             "qcl6": ("mfma,mfma64,zform", "zform", 400),
             "sweep6": ("mfma,mfma64,zform", "zform", 400),
             "maxwell": ("mixed", "", 0)}
 MFMA_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA
-# v_mfma_f32_16x16x32_f16 per 16-cell block and step: 3 split products x
-# M-tiles x K chunks (six levels: 3 x 3 x 2)
+# v_mfma_f32_16x16x32_f16 per 16-cell block and step: split products (3, or
+# 4 with W - I in three pieces for three levels) x M-tiles x K chunks (six
+# levels: 3 x 3 x 2)
 MFMA_PER_BLOCK = {6: 18, 5: 6, 4: 3, 3: 4}
 MFMA_FLOPS = 2 * 16 * 16 * 32
 
@@ -406,15 +407,19 @@ def main():
         if ref is not None else None,
     }
     if head.startswith("mfma"):
-        # matrix-core work of the uniform tiles (the few tiles that straddle
-        # runs or the grid ends take the VALU kernel)
+        # matrix-core work of the tiles of one material with a density (the
+        # tiles that straddle runs, regions or the grid ends take the VALU
+        # z-form, vacuum tiles have no density)
         lev = {"qcl6": 6, "sweep6": 6, "v3": 3}.get(args.config, 0)
-        blocks = 4 * info["num_tiles"] * launches * info["temporal_block"]
+        blocks = 4 * info["mfma_tiles"] * launches * info["temporal_block"]
         hw = blocks * MFMA_PER_BLOCK.get(lev, 0) * MFMA_FLOPS
         t = hw / (r["dev_ms"] * 1e-3) / 1e12
         out["mfma"] = {"hardware_tflops": t, "peak_tflops": MFMA_PEAK_TFLOPS,
                        "frac": t / MFMA_PEAK_TFLOPS,
-                       "instr": "v_mfma_f32_16x16x32_f16, 3 split products"}
+                       "mfma_per_tile_step": 4 * MFMA_PER_BLOCK.get(lev, 0),
+                       "instr": "v_mfma_f32_16x16x32_f16, split FP16 "
+                                "products (W - I in %d pieces)"
+                                % (3 if lev <= 3 else 2)}
     traffic_file = os.path.join(ROOT, "profiles", "traffic_%s_%s.json"
                                 % (args.config, head))
     if os.path.exists(traffic_file):

@@ -248,7 +248,9 @@ typedef struct mbh_info {
     uint64_t state_bytes;  /* device bytes of one state buffer */
     uint32_t graph_launches;
     uint32_t step_order;   /* MBH_ORDER_* */
-    uint32_t reserved[2];
+    uint32_t mfma_tiles;   /* MFMA forms: tiles whose density update runs on
+                              the matrix cores (one material with a density) */
+    uint32_t reserved[1];
 } mbh_info;
 
 typedef struct mbh_ctx mbh_ctx;

@@ -76,7 +76,7 @@ class mbh_info(C.Structure):
                 ("num_tiles", C.c_uint64), ("total_cells", C.c_uint64),
                 ("launches", C.c_uint64), ("state_bytes", C.c_uint64),
                 ("graph_launches", C.c_uint32), ("step_order", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("mfma_tiles", C.c_uint32), ("reserved", C.c_uint32 * 1)]
 
 
 _lib = None

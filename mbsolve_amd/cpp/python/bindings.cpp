@@ -118,6 +118,7 @@ py::dict info_dict(const mbh_info& i)
     d["state_bytes"] = i.state_bytes;
     d["graph_launches"] = i.graph_launches;
     d["step_order"] = i.step_order;
+    d["mfma_tiles"] = i.mfma_tiles;
     return d;
 }
 

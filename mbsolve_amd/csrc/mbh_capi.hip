@@ -369,6 +369,8 @@ struct mbh_ctx {
     bool split = false;
     uint32_t* tiles_a = nullptr;
     uint64_t n_tiles_mixed = 0;
+    uint64_t n_tiles_mfma = 0;      /* uniform tiles of a density material */
+    std::vector<uint8_t> mat_has_qm; /* per material (MFMA forms) */
     dev_source* src = nullptr;
     uint32_t* tile_src = nullptr;
     dev_record* rec = nullptr;
@@ -723,6 +725,7 @@ int build_tile_tables(mbh_ctx* ctx, bool with_records)
     ctx->tile_src = ctx->tile_rec_ptr = ctx->tile_rec_idx = nullptr;
     ctx->tiles_a = nullptr;
     ctx->n_tiles_mixed = 0;
+    ctx->n_tiles_mfma = 0;
     int rc;
     if (ctx->split) {
         /* a tile is uniform when all its 64 cells lie in the grid and have
@@ -739,6 +742,9 @@ int build_tile_tables(mbh_ctx* ctx, bool with_records)
                 }
             }
             (uni ? la : lb).push_back((uint32_t)t);
+            if (uni && ctx->mat_has_qm[(size_t)ctx->mat_h[lo]]) {
+                ctx->n_tiles_mfma++;
+            }
         }
         ctx->n_tiles_mixed = lb.size();
         lb.insert(lb.end(), la.begin(), la.end());
@@ -1318,6 +1324,10 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     if (c->form == MBH_FORM_MFMA || c->form == MBH_FORM_MFMA64) {
         c->split = true;
         c->mat_h = mat;
+        c->mat_has_qm.resize(qm.size() / std::max<uint64_t>(1, c->qstride));
+        for (size_t mi = 0; mi < c->mat_has_qm.size(); mi++) {
+            c->mat_has_qm[mi] = qm[mi * c->qstride] != 0.0 ? 1 : 0;
+        }
     }
 
     /* launch geometry: measured, then the final per-tile tables */
@@ -1547,6 +1557,7 @@ int mbh_get_info(const mbh_ctx* ctx, mbh_info* info)
     info->state_bytes = ctx->state_bytes;
     info->graph_launches = ctx->graph_launches;
     info->step_order = ctx->order;
+    info->mfma_tiles = (uint32_t)ctx->n_tiles_mfma;
     return MBH_OK;
 }
 

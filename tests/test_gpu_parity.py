@@ -206,6 +206,18 @@ def test_mfma_batch_mixed_tiles(gpu):
     runs = [setups.build(s, clear_library=False) for s in specs]
     print("case mfma batch", flush=True)
     batch = mb.solver_hip_batch("hip-fdtd-6lvl-rodr-mfma", runs)
+    info = batch.info()
+    # MFMA tiles: the loaded span (owned cells plus the halo) lies inside
+    # one run (runs follow each other with one pseudo cell in between)
+    own, halo = info["owned_cells"], info["temporal_block"]
+    spans, base = [], 0
+    for sp in specs:
+        spans.append((base, base + sp["num_gridpoints"]))
+        base += sp["num_gridpoints"] + 1
+    expect = sum(1 for t in range(info["num_tiles"])
+                 if any(b0 <= t * own - halo and t * own - halo + 64 <= b1
+                        for b0, b1 in spans))
+    assert info["mfma_tiles"] == expect, (info, expect)
     batch.run()
     for r in range(len(specs)):
         ref = oracle.OracleRun(batch.problem(r)).run().records()
