@@ -233,13 +233,17 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
     }
 }
 
-/* ZM_WAVES_PER_EU (build option) asks the compiler for that many
- * wavefronts per SIMD, spilling if it has to */
+/* Wavefronts per SIMD the compiler is asked to fit: two for N = 3 (179
+ * registers instead of 248; 2 % faster on 65536 cells); none for N >= 4
+ * (asked for two, four levels fit in 211 registers but ran 30 % slower;
+ * five and six levels would spill).  ZM_WAVES_PER_EU (build option)
+ * overrides it for every N. */
 #ifdef ZM_WAVES_PER_EU
 #define ZM_OCCUPANCY \
     __attribute__((amdgpu_waves_per_eu(ZM_WAVES_PER_EU, ZM_WAVES_PER_EU)))
 #else
-#define ZM_OCCUPANCY
+#define ZM_OCCUPANCY \
+    __attribute__((amdgpu_waves_per_eu((N <= 3) ? 2 : 1)))
 #endif
 
 template<int N, typename ST>
@@ -476,11 +480,14 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
                     case MBH_REC_MAGNETIC:
                         v = h;
                         break;
+                    /* without a density (vacuum) d = 0: inversion 0,
+                     * populations 1 / N, as calc_population of the zero
+                     * coherence vector */
                     case MBH_REC_INVERSION:
-                        v = my_qm ? dd[N * (N - 1)] : 0.0;
+                        v = dd[N * (N - 1)];
                         break;
                     default:
-                        v = my_qm ? cv_population<N>(dd, R.row_idx) : 0.0;
+                        v = cv_population<N>(dd, R.row_idx);
                         break;
                     }
                     R.dst[row * R.cols + (uint64_t)(gl - (int64_t)R.gbegin)] =

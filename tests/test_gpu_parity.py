@@ -156,7 +156,10 @@ def _qcl6_with_populations():
     # rounding accumulates faster
     (lambda: setups.v3_rodr(num_gridpoints=2048, endtime=0.9e-12,
                             num_records=32), 5e-6),
-], ids=["qcl6", "marskar4", "marskar5", "v3"])
+    # vacuum margins: uniform tiles without a density record populations
+    # 1/N like the reference's zero coherence vector
+    (lambda: _v3_three_regions(), 5e-6),
+], ids=["qcl6", "marskar4", "marskar5", "v3", "v3_regions"])
 def test_mfma_forms_vs_oracle(gpu, form, make, tol):
     """Rodrigues z-form with the generator product on MFMA (split FP16,
     FP32 accumulation) against the oracle: fields relative to their
@@ -258,3 +261,14 @@ def test_red_order_independent_of_tiling(gpu, tb, tile):
     _, recs = _run_hip(spec, order="red", temporal_block=tb, tile_cells=tile)
     for a, b in zip(recs, base):
         assert np.array_equal(a, b)
+
+
+def _v3_three_regions():
+    """three-level medium between vacuum margins with point and full-grid
+    population records: tiles at the region boundaries take the VALU body"""
+    spec = setups.v3_rodr(num_gridpoints=1500, endtime=0.4e-12, num_records=9)
+    spec["records"].append({"name": "d22", "type": 3, "row": 2, "col": 2,
+                            "interval": spec["endtime"] / 8})
+    spec["records"].append({"name": "d33", "type": 3, "row": 3, "col": 3,
+                            "interval": 0.0, "position": 75e-6})
+    return spec
