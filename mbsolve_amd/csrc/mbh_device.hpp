@@ -187,6 +187,12 @@ struct algo_noop {
     }
 };
 
+/* noop with the folded FDTD constants (FP32-storage form: not bound to the
+ * reference's operation order) */
+struct algo_noop_fast : algo_noop {
+    static constexpr bool FAST_FDTD = true;
+};
+
 /* ======================================================================
  * lindblad_reg_cayley<N>::update (algo_lindblad_reg_cayley.hpp:228-257)
  *

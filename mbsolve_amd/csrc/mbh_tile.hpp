@@ -377,7 +377,9 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 #pragma unroll
         for (int c = 0; c < C; c++) {
             const double hn = (c + 1 < C) ? h[(c + 1) % C] : h_right;
-            if constexpr (A::FAST_FDTD) {
+            if constexpr (A::FAST_FDTD && A::DC == 0) {
+                en[c] = fma(fx[c], hn - h[c], fa[c] * e[c]); /* p = 0 */
+            } else if constexpr (A::FAST_FDTD) {
                 en[c] = fma(fx[c], hn - h[c], fma(fg[c], p[c], fa[c] * e[c]));
             } else {
                 en[c] = (m[c] >= 0)
