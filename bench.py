@@ -425,9 +425,11 @@ def main():
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             t = json.load(f)
-        if (t.get("temporal_block") == info["temporal_block"] and
-                t.get("tile_cells") == info["tile_cells"]):
-            out["roofline"]["traffic"] = t.get("bytes_per_launch")
+        # one measured geometry, or a list of them under "geometries"
+        for g in t.get("geometries", [t]):
+            if (g.get("temporal_block") == info["temporal_block"] and
+                    g.get("tile_cells") == info["tile_cells"]):
+                out["roofline"]["traffic"] = g.get("bytes_per_launch")
     # the figure mbsolve-tool prints (mbsolve-tool.cpp:693-695): N_x T / dt
     # per run time, i.e. n_t - 1 counted steps per n_t run steps, here for
     # a full run at the measured per-step rate
