@@ -286,17 +286,45 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
         }
     }
 
-    /* material: one uniform material for the whole tile (the common case)
-     * is decided once per launch */
-    const int32_t m0 = __builtin_amdgcn_readfirstlane(m[0]);
-    bool same = true;
+    /* material with a density, per cell (-1: vacuum or outside the grid).
+     * The common case is one such material m1 for every cell of the tile
+     * that has a density: then the update runs with m1's constants as in a
+     * uniform tile, on the lanes whose cells are all of m1 (a lane at an
+     * interface between m1 and a material without density updates its m1
+     * cells one by one). Tiles with two or more density materials take one
+     * update per material. The arithmetic of a cell is the same in every
+     * case; only the number of passes differs. */
+    int32_t ma[C];
+    int32_t first = -1;
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        same = same && (m[c] == m0);
+        bool act = false;
+        if constexpr (A::DC > 0) {
+            if (m[c] >= 0) {
+                act = A::has_qm((cdp)k.qm + (uint64_t)m[c] * k.qstride);
+            }
+        }
+        ma[c] = act ? m[c] : -1;
     }
-    const bool uniform = __all(same);
-    cdp q0 = (cdp)k.qm + (uint64_t)(m0 < 0 ? 0 : m0) * k.qstride;
-    const bool q0_active = uniform && m0 >= 0 && A::DC > 0 && A::has_qm(q0);
+#pragma unroll
+    for (int c = C - 1; c >= 0; c--) {
+        first = (ma[c] >= 0) ? ma[c] : first;
+    }
+    const unsigned long long act_lanes = __ballot(first >= 0);
+    const int32_t m1 = act_lanes
+        ? __builtin_amdgcn_readlane(first, __builtin_ctzll(act_lanes))
+        : -1;
+    bool only1 = true, all1 = true, any1 = false;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        only1 = only1 && (ma[c] < 0 || ma[c] == m1);
+        all1 = all1 && (ma[c] == m1);
+        any1 = any1 || (ma[c] == m1);
+    }
+    const bool single = (m1 >= 0) && __all(only1);
+    const bool full1 = single && __all(all1);
+    cdp q0 = (cdp)k.qm + (uint64_t)(m1 < 0 ? 0 : m1) * k.qstride;
+    const bool q0_active = single;
     /* doubles of the material blob the algorithm wants staged in LDS */
     constexpr int STAGE = A::STAGE;
     __shared__ double s_q[STAGE > 0 ? STAGE : 1];
@@ -338,7 +366,7 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
      * (solver_cpu_fdtd.tpp:278-285, solver_cpu_fdtd_red.tpp:451-457) */
     auto density_update = [&]() {
         if constexpr (A::DC > 0) {
-            if (q0_active) {
+            auto update_m1 = [&]() {
                 if constexpr (STAGE > 0) {
                     /* an offset the compiler cannot see through keeps the
                      * LDS reads inside the step (hoisted, the staged
@@ -350,14 +378,27 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
                 } else {
                     A::template update_multi<C>(q0, d, e, p);
                 }
-            } else if (!uniform) {
+            };
+            if (q0_active) {
+                /* every cell of m1: a wave-uniform branch, full exec */
+                if (full1) {
+                    update_m1();
+                } else if (all1) {
+                    update_m1();
+                } else if (any1) {
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        if (ma[c] == m1) {
+                            A::update(q0, d[c], e[c], p[c]);
+                        }
+                    }
+                }
+            } else if (m1 >= 0) {
 #pragma unroll
                 for (int c = 0; c < C; c++) {
-                    for_each_material(m[c], [&](int32_t mm) {
-                        cdp q = (cdp)k.qm + (uint64_t)mm * k.qstride;
-                        if (A::has_qm(q)) {
-                            A::update(q, d[c], e[c], p[c]);
-                        }
+                    for_each_material(ma[c], [&](int32_t mm) {
+                        A::update((cdp)k.qm + (uint64_t)mm * k.qstride, d[c],
+                                  e[c], p[c]);
                     });
                 }
             }

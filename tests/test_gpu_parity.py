@@ -64,6 +64,51 @@ def test_fp64_independent_of_tiling(gpu, tb, tile):
         assert np.array_equal(a, b)
 
 
+def _interfaces(case):
+    if case == "gain_absorber":
+        spec = setups.tzenov2018_cpml(num_gridpoints=1200, endtime=2e-12)
+        iv = spec["endtime"] / 16
+        spec["records"] = [{"name": "inv12", "interval": iv},
+                           {"name": "e", "interval": iv},
+                           {"name": "d22", "type": 3, "row": 2, "col": 2,
+                            "interval": iv}]
+        return spec
+    if case == "sit2":
+        return setups.sit_2lvl(num_gridpoints=1300, endtime=0.4e-12,
+                               num_records=16)
+    return setups.v3_rodr(num_gridpoints=1300, endtime=0.3e-12,
+                          num_records=16)
+
+
+@pytest.mark.parametrize("case,form,tiles,tol", [
+    ("gain_absorber", "", (64, 128, 256), FP64_TOL),
+    ("gain_absorber", "zform", (64, 128, 256, 512), 1e-11),
+    ("sit2", "", (64, 128, 256), FP64_TOL),
+    ("sit2", "zform", (64, 128, 256, 512), 1e-11),
+    ("v3", "", (64, 128), FP64_TOL),
+    ("v3", "zform", (64, 128), 1e-10),
+])
+def test_interface_tiles(gpu, case, form, tiles, tol):
+    """Tiles at material interfaces: one density material beside vacuum
+    (at two or more cells per lane some lanes straddle the interface) and
+    two density materials in one tile (gain | absorber | gain). Every
+    tiling matches the oracle; the FP64 form is bitwise the same for
+    every tiling."""
+    ref = base = None
+    for tile in tiles:
+        sol, recs = _run_hip(_interfaces(case), form, temporal_block=8,
+                             tile_cells=tile)
+        if ref is None:
+            ref = _oracle_records(sol)
+        for a, b in zip(recs, ref):
+            assert rel_err(a, b) <= tol, (case, form, tile, rel_err(a, b))
+        if form == "":
+            if base is None:
+                base = recs
+            for a, b in zip(recs, base):
+                assert np.array_equal(a, b), (case, tile)
+
+
 def test_mixed_rodr_within_tolerance(gpu):
     spec = setups.v3_rodr(num_gridpoints=2048, endtime=0.9e-12,
                           num_records=32)
