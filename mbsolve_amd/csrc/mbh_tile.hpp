@@ -286,14 +286,13 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
         }
     }
 
-    /* material with a density, per cell (-1: vacuum or outside the grid).
-     * The common case is one such material m1 for every cell of the tile
-     * that has a density: then the update runs with m1's constants as in a
-     * uniform tile, on the lanes whose cells are all of m1 (a lane at an
-     * interface between m1 and a material without density updates its m1
-     * cells one by one). Tiles with two or more density materials take one
-     * update per material. The arithmetic of a cell is the same in every
-     * case; only the number of passes differs. */
+    /* material with a density, per cell (-1: vacuum or outside the grid),
+     * and m1, the first of them in the tile. The lanes whose cells all have
+     * m1 update with m1's constants as in a tile of one material (a tile
+     * at an interface between m1 and vacuum is then as fast as one inside
+     * the medium); the other lanes of the tile (a cell of another density
+     * material, or one lane straddling the interface) take one update per
+     * material. The arithmetic of a cell is the same either way. */
     int32_t ma[C];
     int32_t first = -1;
 #pragma unroll
@@ -314,17 +313,15 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
     const int32_t m1 = act_lanes
         ? __builtin_amdgcn_readlane(first, __builtin_ctzll(act_lanes))
         : -1;
-    bool only1 = true, all1 = true, any1 = false;
+    bool all1 = true;
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        only1 = only1 && (ma[c] < 0 || ma[c] == m1);
         all1 = all1 && (ma[c] == m1);
-        any1 = any1 || (ma[c] == m1);
     }
-    const bool single = (m1 >= 0) && __all(only1);
-    const bool full1 = single && __all(all1);
+    all1 = all1 && (m1 >= 0);
+    const bool full1 = __all(all1);
     cdp q0 = (cdp)k.qm + (uint64_t)(m1 < 0 ? 0 : m1) * k.qstride;
-    const bool q0_active = single;
+    const bool q0_active = m1 >= 0;
     /* doubles of the material blob the algorithm wants staged in LDS */
     constexpr int STAGE = A::STAGE;
     __shared__ double s_q[STAGE > 0 ? STAGE : 1];
@@ -379,21 +376,19 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
                     A::template update_multi<C>(q0, d, e, p);
                 }
             };
-            if (q0_active) {
-                /* every cell of m1: a wave-uniform branch, full exec */
-                if (full1) {
-                    update_m1();
-                } else if (all1) {
-                    update_m1();
-                } else if (any1) {
-#pragma unroll
-                    for (int c = 0; c < C; c++) {
-                        if (ma[c] == m1) {
-                            A::update(q0, d[c], e[c], p[c]);
-                        }
-                    }
-                }
+            /* small updates (two levels) get a second copy behind a
+             * wave-uniform branch, so that tiles of one material run it
+             * with full exec and no per-lane branch; for the large ones
+             * the second copy costs more (instruction cache) than the
+             * per-lane branch */
+            constexpr bool TWO_COPIES = A::DC <= 4;
+            if (TWO_COPIES && full1) {
+                update_m1();
+            } else if (all1) {
+                update_m1();
             } else if (m1 >= 0) {
+                /* lanes with a cell of another material: one update per
+                 * material */
 #pragma unroll
                 for (int c = 0; c < C; c++) {
                     for_each_material(ma[c], [&](int32_t mm) {
