@@ -93,18 +93,61 @@ struct zm_geom {
     }
 };
 
-/* Lane (q, j) holds v[n] for blocks n = 0..3; returns sum over the four
- * lane groups of v[q] (block n = q, i.e. the cell of this lane in the
- * one-cell-per-lane layout).  Two exchange rounds (xor 32, xor 16). */
-__device__ __forceinline__ double reduce_scatter4(const double* v, int q)
+/* v_permlane16_swap / v_permlane32_swap on both 32-bit halves of a double
+ * (VALU lane moves, no LDS round trip). With rows of 16 lanes:
+ * swap16(a, b): a = [a0 b0 a2 b2], b = [a1 b1 a3 b3];
+ * swap32(a, b): a = [a0 a1 b0 b1], b = [a2 a3 b2 b3]. */
+__device__ __forceinline__ void swap16(double& a, double& b)
 {
-    const bool b = (q & 2) != 0, c = (q & 1) != 0;
-    const double keep0 = b ? v[2] : v[0], keep1 = b ? v[3] : v[1];
-    const double send0 = b ? v[0] : v[2], send1 = b ? v[1] : v[3];
-    const double s0 = keep0 + __shfl_xor(send0, 32);
-    const double s1 = keep1 + __shfl_xor(send1, 32);
-    const double keep = c ? s1 : s0, send = c ? s0 : s1;
-    return keep + __shfl_xor(send, 16);
+    const auto lo = __builtin_amdgcn_permlane16_swap(
+        (unsigned)__double2loint(a), (unsigned)__double2loint(b), false,
+        false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(
+        (unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false,
+        false);
+    a = __hiloint2double((int)hi[0], (int)lo[0]);
+    b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+__device__ __forceinline__ void swap32(double& a, double& b)
+{
+    const auto lo = __builtin_amdgcn_permlane32_swap(
+        (unsigned)__double2loint(a), (unsigned)__double2loint(b), false,
+        false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(
+        (unsigned)__double2hiint(a), (unsigned)__double2hiint(b), false,
+        false);
+    a = __hiloint2double((int)hi[0], (int)lo[0]);
+    b = __hiloint2double((int)hi[1], (int)lo[1]);
+}
+
+/* row r of x, broadcast to all four rows: out[r] */
+__device__ __forceinline__ void rows_broadcast(double x, double* out)
+{
+    double a = x, b = x;
+    swap32(a, b); /* a = [x0 x1 x0 x1], b = [x2 x3 x2 x3] */
+    double c = a, d = b;
+    swap16(a, c); /* a = [x0 x0 x0 x0], c = [x1 x1 x1 x1] */
+    swap16(b, d); /* b = [x2 x2 x2 x2], d = [x3 x3 x3 x3] */
+    out[0] = a;
+    out[1] = c;
+    out[2] = b;
+    out[3] = d;
+}
+
+/* Lane (q, j) holds v[n] for blocks n = 0..3; returns the sum over the
+ * four rows of v[q] (block n = q, i.e. the cell of this lane in the
+ * one-cell-per-lane layout): halves first, then neighbouring rows, each
+ * sum adding the same two operands as an exchange of lanes xor 32, then
+ * xor 16, would. */
+__device__ __forceinline__ double reduce_scatter4(const double* v)
+{
+    double a0 = v[0], a2 = v[2], a1 = v[1], a3 = v[3];
+    swap32(a0, a2); /* a0 = [v0 v0' ...]: lower half v0, upper half v2 */
+    swap32(a1, a3);
+    double s0 = a0 + a2; /* rows 0, 1: v0 over the halves; 2, 3: v2 */
+    double s1 = a1 + a3; /* rows 0, 1: v1; rows 2, 3: v3 */
+    swap16(s0, s1);
+    return s0 + s1;
 }
 
 /* per-material constants of one lane */
@@ -385,10 +428,8 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
 
         /* density update */
         double eb[Gm::NB];
-#pragma unroll
-        for (int nb = 0; nb < Gm::NB; nb++) {
-            eb[nb] = __shfl(e, 16 * nb + j);
-        }
+        static_assert(Gm::NB == 4, "four 16-lane rows per wavefront");
+        rows_broadcast(e, eb);
         if (q0_active) {
             double emax = fabs(eb[0]);
 #pragma unroll
@@ -409,7 +450,7 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
             } else {
                 zm_step<N, 3>(KC0, s_frag, lane, eb, one_q0, Y, pp);
             }
-            p = reduce_scatter4(pp, q) + KC0.b;
+            p = reduce_scatter4(pp) + KC0.b;
         }
 
         /* record copy */
@@ -440,7 +481,7 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
                         }
                         part[nb] = acc;
                     }
-                    double v = reduce_scatter4(part, q);
+                    double v = reduce_scatter4(part);
                     if (my_qm) {
                         v += reinterpret_cast<const double*>(
                             fbase + (uint64_t)m * k.frag_stride +
