@@ -15,9 +15,11 @@ namespace mbh {
 template __global__ void tile_kernel<algo_noop, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_noop, 2, double>(KArgs);
 template __global__ void tile_kernel<algo_noop, 4, double>(KArgs);
+template __global__ void tile_kernel<algo_noop, 8, double>(KArgs);
 template __global__ void tile_kernel<algo_noop_fast, 1, float>(KArgs);
 template __global__ void tile_kernel<algo_noop_fast, 2, float>(KArgs);
 template __global__ void tile_kernel<algo_noop_fast, 4, float>(KArgs);
+template __global__ void tile_kernel<algo_noop_fast, 8, float>(KArgs);
 template __global__ void tile_kernel<algo_cayley<2>, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<2>, 2, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<2>, 4, double>(KArgs);
@@ -44,9 +46,11 @@ const kernel_entry k_table_basic[] = {
     MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_FP64, 1, algo_noop, double),
     MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_FP64, 2, algo_noop, double),
     MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_FP64, 4, algo_noop, double),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_FP64, 8, algo_noop, double),
     MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 1, algo_noop_fast, float),
     MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 2, algo_noop_fast, float),
     MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 4, algo_noop_fast, float),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 8, algo_noop_fast, float),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 1, algo_cayley<2>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 2, algo_cayley<2>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 4, algo_cayley<2>, double),

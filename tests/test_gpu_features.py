@@ -160,6 +160,39 @@ def test_noop_maxwell(gpu):
             assert rel_err(r.get_data_real(), b) <= tol
 
 
+@pytest.mark.parametrize("case,form,tile", [
+    ("sit2", "", 256), ("sit2", "zform", 512), ("noop", "", 512),
+    ("noop", "mixed", 512)])
+def test_deepest_temporal_block(gpu, case, form, tile):
+    """Temporal block 64 (the deepest, one bit per step in the record mask)
+    and a request beyond it (clamped to 64): records every step, equal to
+    the one-step-per-launch run (bitwise for the reference-order FP64 form,
+    to the form's bound otherwise)."""
+    if case == "sit2":
+        spec = setups.sit_2lvl(num_gridpoints=2000, endtime=0.2e-12,
+                               num_records=100000)
+    else:
+        spec = setups.maxwell_line(num_gridpoints=40000, num_timesteps=1200)
+        spec["records"] = [{"name": "e", "interval": 0.0,
+                            "position": 300e-9},
+                           {"name": "h", "interval": 0.0,
+                            "position": 400e-9}]
+    base = _solve(spec, form, temporal_block=1, tile_cells=64)
+    ref = [r.get_data_real() for r in base.get_results()]
+    assert ref[0].shape[0] > 128
+    for tb in (64, 100):
+        sol = _solve(spec, form, temporal_block=tb, tile_cells=tile)
+        assert sol.info()["temporal_block"] == 64
+        for r, b in zip(sol.get_results(), ref):
+            a = r.get_data_real()
+            assert np.max(np.abs(b)) > 0.0
+            if form == "":
+                assert np.array_equal(a, b), (case, tb)
+            else:
+                tol = 1e-11 if form == "zform" else 1e-5
+                assert rel_err(a, b) <= tol, (case, form, tb, rel_err(a, b))
+
+
 def test_state_after_partial_runs(gpu):
     """run_steps in pieces equals one call; fetch_state is the oracle's."""
     spec = setups.sit_2lvl(num_gridpoints=900, endtime=0.3e-12,
