@@ -368,6 +368,7 @@ struct mbh_ctx {
      * tiles first so that their longer waves start early */
     bool split = false;
     uint32_t* tiles_a = nullptr;
+    int32_t* tiles_mat = nullptr; /* per list position: material | QM flag */
     uint64_t n_tiles_mixed = 0;
     uint64_t n_tiles_mfma = 0;      /* uniform tiles of a density material */
     std::vector<uint8_t> mat_has_qm; /* per material (MFMA forms) */
@@ -722,8 +723,10 @@ int build_tile_tables(mbh_ctx* ctx, bool with_records)
     release(ctx->tile_rec_ptr);
     release(ctx->tile_rec_idx);
     release(ctx->tiles_a);
+    release(ctx->tiles_mat);
     ctx->tile_src = ctx->tile_rec_ptr = ctx->tile_rec_idx = nullptr;
     ctx->tiles_a = nullptr;
+    ctx->tiles_mat = nullptr;
     ctx->n_tiles_mixed = 0;
     ctx->n_tiles_mfma = 0;
     int rc;
@@ -748,7 +751,16 @@ int build_tile_tables(mbh_ctx* ctx, bool with_records)
         }
         ctx->n_tiles_mixed = lb.size();
         lb.insert(lb.end(), la.begin(), la.end());
+        std::vector<int32_t> lm(lb.size(), -1);
+        for (size_t i = ctx->n_tiles_mixed; i < lb.size(); i++) {
+            const int64_t lo = (int64_t)(lb[i] * ctx->owned) - ctx->halo;
+            const int32_t m0 = ctx->mat_h[lo];
+            lm[i] = m0 | (ctx->mat_has_qm[(size_t)m0] ? TILE_MAT_QM : 0);
+        }
         if ((rc = upload(ctx, &ctx->tiles_a, lb.data(), lb.size()))) {
+            return rc;
+        }
+        if ((rc = upload(ctx, &ctx->tiles_mat, lm.data(), lm.size()))) {
             return rc;
         }
     }
@@ -918,6 +930,8 @@ KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
 hipError_t launch_block(const mbh_ctx* ctx, KArgs k)
 {
     k.tile_list = ctx->split ? ctx->tiles_a : nullptr;
+    k.tile_mat = ctx->split ? ctx->tiles_mat : nullptr;
+    k.n_mixed = ctx->split ? (uint32_t)ctx->n_tiles_mixed : 0u;
     return ctx->launch(k, ctx->num_tiles, ctx->stream);
 }
 

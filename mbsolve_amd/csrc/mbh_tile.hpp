@@ -87,7 +87,14 @@ struct KArgs {
     const void* frag;     /* MFMA form: [nmat][frag_stride bytes] */
     uint64_t frag_stride;
     const uint32_t* tile_list; /* tiles of this launch (NULL: blockIdx) */
+    /* MFMA form: the first n_mixed tiles of the list hold more than one
+     * material; tile_mat[i] (i: position in the list) is the material of
+     * a uniform tile, plus TILE_MAT_QM when it has a density (NULL: the
+     * kernel reads the cells' materials) */
+    const int32_t* tile_mat;
+    uint32_t n_mixed;
 };
+constexpr int32_t TILE_MAT_QM = 0x40000000;
 
 template<typename ST>
 __device__ __forceinline__ double ld(const void* p, uint64_t i)

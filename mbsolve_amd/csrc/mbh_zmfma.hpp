@@ -299,18 +299,17 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
     const int q = lane >> 4, j = lane & 15;
     const float one_q0 = (q == 0) ? 1.0f : 0.0f;
     const uint64_t t = k.tile_list ? k.tile_list[blockIdx.x] : blockIdx.x;
-    {
-        /* a tile whose 64 cells are not all of one material runs the VALU
-         * z-form body (same state layout); such tiles come first in the
-         * tile list, so they overlap with the MFMA tiles */
-        const int64_t gc = (int64_t)(t * k.owned) - (int64_t)k.halo + lane;
-        const int32_t mc = (gc >= 0 && gc < (int64_t)k.G) ? k.mat[gc] : -1;
-        const int32_t mc0 = __builtin_amdgcn_readfirstlane(mc);
-        if (!__all(mc == mc0)) {
-            tile_body<algo_zform<N>, 1, ST>(k, t);
-            return;
-        }
+    /* a tile whose 64 cells are not all of one material runs the VALU
+     * z-form body (same state layout); such tiles come first in the tile
+     * list, so they overlap with the MFMA tiles. With the host's table the
+     * decision and the tile's material need no load of the cells'
+     * materials, and the constant loads start at once. */
+    if (!k.tile_mat || blockIdx.x < k.n_mixed) {
+        /* (without the table every tile takes the VALU body) */
+        tile_body<algo_zform<N>, 1, ST>(k, t);
+        return;
     }
+    const int32_t tm = as_const(k.tile_mat)[blockIdx.x];
     const int64_t own_lo = (int64_t)(t * k.owned);
     const int64_t own_hi =
         ((own_lo + (int64_t)k.owned) < (int64_t)k.Gown) ? own_lo + k.owned
@@ -322,7 +321,8 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
 
     /* FDTD state, one cell per lane (folded constants as in tile_kernel) */
     const bool in = (gl >= 0) && (gl < (int64_t)k.G);
-    const int32_t m = in ? k.mat[gl] : -1;
+    /* a uniform tile lies inside the grid: every lane has the material */
+    const int32_t m = tm & ~TILE_MAT_QM;
     const bool hz = in ? (k.hzero[gl] != 0) : true;
     double e = in ? ld<ST>(k.e_in, gl) : 0.0;
     double h = in ? ld<ST>(k.h_in, gl) : 0.0;
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
     for (int n = 0; n < Gm::NB; n++) {
         const int64_t gn = g0 + 16 * n + j;
         const bool inn = (gn >= 0) && (gn < (int64_t)k.G);
-        mb[n] = __shfl(m, 16 * n + j);
+        mb[n] = m;
 #pragma unroll
         for (int u = 0; u < Gm::ROWS; u++) {
             const int row = Gm::slot_row(q, u);
@@ -354,24 +354,16 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
         }
     }
 
-    /* material constants: one uniform material (the common case) is
-     * loaded once; mixed tiles go through a per-material loop */
-    const int32_t m0 = __builtin_amdgcn_readfirstlane(m);
-    const bool uniform = __all(m == m0);
+    /* material constants of the tile (one material, from the table) */
+    const int32_t m0 = m;
     const char* fbase = static_cast<const char*>(k.frag);
-    const bool q0_active = uniform && m0 >= 0 &&
-        as_const(reinterpret_cast<const double*>(
-            fbase + (uint64_t)m0 * k.frag_stride))[3] != 0.0;
+    const bool q0_active = (tm & TILE_MAT_QM) != 0;
     __shared__ zm_h8 s_frag[Gm::NF][64];
     zm_consts<N> KC0;
     if (q0_active) {
         KC0.load(fbase + (uint64_t)m0 * k.frag_stride, lane, s_frag);
     }
-    bool my_qm = false; /* this lane's cell has a density */
-    if (m >= 0) {
-        my_qm = reinterpret_cast<const double*>(
-                    fbase + (uint64_t)m * k.frag_stride)[3] != 0.0;
-    }
+    const bool my_qm = q0_active; /* this lane's cell has a density */
 
     /* sources of this tile */
     const uint32_t s_lo = as_const(k.tile_src)[2 * t];
