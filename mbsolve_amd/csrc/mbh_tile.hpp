@@ -520,8 +520,19 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
     }
 }
 
+/* wavefronts per SIMD the compiler is asked to fit (1: no request): the
+ * two-level Bloch kernel at four cells per lane fits three without spills
+ * in its step loop, against two unasked */
+template<class A, int C>
+constexpr int tile_min_waves()
+{
+    return (std::is_same<A, algo_bloch2>::value && C == 4) ? 3 : 1;
+}
+
 template<class A, int C, typename ST, bool RED = false>
-__global__ __launch_bounds__(64) void tile_kernel(KArgs k)
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(tile_min_waves<A, C>()))) void
+tile_kernel(KArgs k)
 {
     tile_body<A, C, ST, RED>(k, k.tile_list ? k.tile_list[blockIdx.x]
                                             : blockIdx.x);
