@@ -228,7 +228,8 @@ typedef struct mbh_problem {
 
 typedef struct mbh_options {
     uint32_t form;           /* MBH_FORM_* */
-    uint32_t temporal_block; /* time steps per kernel launch, 0 = auto */
+    uint32_t temporal_block; /* time steps per kernel launch, 0 = auto;
+                                at most 64 (a deeper request is clamped) */
     uint32_t tile_cells;     /* cells per wavefront tile incl. halo, 0 = auto */
     int32_t device;          /* HIP device ordinal, -1 = current */
     uint32_t graph_launches; /* kernel launches per captured hipGraph,
