@@ -799,12 +799,12 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
     }
     std::vector<std::pair<uint32_t, uint32_t>> cand;
     for (uint32_t C : cs) {
-        for (uint32_t tb : { 1u, 2u, 4u, 6u, 8u, 12u, 16u, 20u, 24u, 32u,
-                             48u, 64u }) {
+        for (uint32_t tb : { 1u, 2u, 4u, 6u, 8u, 12u, 16u, 20u, 24u, 28u,
+                             32u, 48u, 64u }) {
             if (req_tb && tb != req_tb) {
                 continue;
             }
-            if (tb > MAX_BLOCK || 64u * C < 2u * tb + 16u) {
+            if (tb > MAX_BLOCK || 64u * C < 2u * tb + 8u) {
                 continue;
             }
             if (!req_tb && tb > std::max<uint64_t>(1, ctx->n_t)) {
@@ -818,7 +818,7 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
          * what it and the kernels (one bit per step of a block in the
          * record mask) allow */
         const uint32_t C = cs.back();
-        cand.emplace_back(C, std::min<uint32_t>({req_tb, 32u * C - 8u,
+        cand.emplace_back(C, std::min<uint32_t>({req_tb, 32u * C - 4u,
                                                  (uint32_t)MAX_BLOCK}));
     }
     if (cand.size() == 1) {

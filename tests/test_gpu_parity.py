@@ -54,7 +54,8 @@ def test_fp64_matches_oracle(gpu, case):
         assert rel_err(a, b) <= FP64_TOL, (case, rel_err(a, b))
 
 
-@pytest.mark.parametrize("tb,tile", [(1, 64), (5, 128), (16, 256), (32, 256)])
+@pytest.mark.parametrize("tb,tile", [(1, 64), (5, 128), (16, 256), (32, 256),
+                                     (28, 64)])
 def test_fp64_independent_of_tiling(gpu, tb, tile):
     spec = setups.sit_2lvl(num_gridpoints=1500, endtime=0.5e-12,
                            num_records=16)
