@@ -276,8 +276,9 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
     }
 }
 
-/* Wavefronts per SIMD the compiler is asked to fit: two for N = 3 (179
- * registers instead of 248; 2 % faster on 65536 cells); none for N >= 4
+/* Wavefronts per SIMD the compiler is asked to fit: two for N = 3 (165
+ * registers instead of 212 unasked; 2 % faster on 65536 cells when
+ * measured against the unasked build); none for N >= 4
  * (asked for two, four levels fit in 211 registers but ran 30 % slower;
  * five and six levels would spill).  ZM_WAVES_PER_EU (build option)
  * overrides it for every N. */
