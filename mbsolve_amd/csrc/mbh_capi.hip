@@ -367,6 +367,8 @@ struct mbh_ctx {
      * the VALU z-form body in the same kernel; tiles_a lists the mixed
      * tiles first so that their longer waves start early */
     bool split = false;
+    /* every cell with H held at zero starts with H = 0 */
+    bool hz_zero = false;
     uint32_t* tiles_a = nullptr;
     int32_t* tiles_mat = nullptr; /* per list position: material | QM flag */
     uint64_t n_tiles_mixed = 0;
@@ -785,7 +787,11 @@ hipError_t launch_block(const mbh_ctx* ctx, KArgs k);
  */
 int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
 {
-    const uint32_t kform = kernel_form(ctx->algo, ctx->levels, ctx->form);
+    uint32_t kform = kernel_form(ctx->algo, ctx->levels, ctx->form);
+    if (ctx->algo == MBH_ALGO_NOOP && kform == MBH_FORM_MIXED &&
+        ctx->hz_zero) {
+        kform = KFORM_NOOP_HZ0;
+    }
     std::vector<uint32_t> cs;
     for (uint32_t C : { 1u, 2u, 4u, 8u }) {
         if (find_kernel(ctx->algo, ctx->levels, kform, C, ctx->order) &&
@@ -1218,6 +1224,13 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
                 }
             }
             h[L.base + L.cells] = p.h_init[L.cells];
+        }
+        c->hz_zero = true;
+        for (uint64_t g = 0; g < G; g++) {
+            if (hz[g] && h[g] != 0.0) {
+                c->hz_zero = false;
+                break;
+            }
         }
         if (c->bloch) {
             /* rho00, rho11 and rho01 of the Hermitian state, converted to

@@ -192,6 +192,9 @@ struct algo_noop {
 struct algo_noop_fast : algo_noop {
     static constexpr bool FAST_FDTD = true;
 };
+/* the same, for states whose cells with H held at zero start at zero: there
+ * fc' = 0 keeps them at zero, so h' = h + fc' (e_c - e_l) needs no factor */
+struct algo_noop_hz0 : algo_noop_fast {};
 
 /* ======================================================================
  * lindblad_reg_cayley<N>::update (algo_lindblad_reg_cayley.hpp:228-257)

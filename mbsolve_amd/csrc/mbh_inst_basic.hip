@@ -20,6 +20,10 @@ template __global__ void tile_kernel<algo_noop_fast, 1, float>(KArgs);
 template __global__ void tile_kernel<algo_noop_fast, 2, float>(KArgs);
 template __global__ void tile_kernel<algo_noop_fast, 4, float>(KArgs);
 template __global__ void tile_kernel<algo_noop_fast, 8, float>(KArgs);
+template __global__ void tile_kernel<algo_noop_hz0, 1, float>(KArgs);
+template __global__ void tile_kernel<algo_noop_hz0, 2, float>(KArgs);
+template __global__ void tile_kernel<algo_noop_hz0, 4, float>(KArgs);
+template __global__ void tile_kernel<algo_noop_hz0, 8, float>(KArgs);
 template __global__ void tile_kernel<algo_cayley<2>, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<2>, 2, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<2>, 4, double>(KArgs);
@@ -51,6 +55,10 @@ const kernel_entry k_table_basic[] = {
     MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 2, algo_noop_fast, float),
     MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 4, algo_noop_fast, float),
     MBH_ENTRY(MBH_ALGO_NOOP, 0, MBH_FORM_MIXED, 8, algo_noop_fast, float),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, KFORM_NOOP_HZ0, 1, algo_noop_hz0, float),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, KFORM_NOOP_HZ0, 2, algo_noop_hz0, float),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, KFORM_NOOP_HZ0, 4, algo_noop_hz0, float),
+    MBH_ENTRY(MBH_ALGO_NOOP, 0, KFORM_NOOP_HZ0, 8, algo_noop_hz0, float),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 1, algo_cayley<2>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 2, algo_cayley<2>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_FP64, 4, algo_cayley<2>, double),

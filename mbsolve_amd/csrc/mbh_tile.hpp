@@ -66,7 +66,7 @@ struct KArgs {
     void* d_out;
     uint64_t G;       /* cells in the super-grid */
     uint64_t Gown;    /* cells owned by tiles: G minus the trailing pseudo
-                         cell, which always holds zeros */
+                         cell (its H is read at step 0 only) */
     uint64_t dstride; /* elements between density components */
     const int32_t* mat;   /* [G], -1: pseudo cell */
     const uint8_t* hzero; /* [G] */
@@ -95,6 +95,8 @@ struct KArgs {
     uint32_t n_mixed;
 };
 constexpr int32_t TILE_MAT_QM = 0x40000000;
+/* kernel-table form of algo_noop_hz0 (internal, beside the MBH_FORM_*) */
+constexpr uint32_t KFORM_NOOP_HZ0 = 0x100;
 
 template<typename ST>
 __device__ __forceinline__ double ld(const void* p, uint64_t i)
@@ -258,7 +260,12 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
         m[c] = in ? k.mat[g] : -1;
         hz[c] = in ? (k.hzero[g] != 0) : true;
         e[c] = in ? ld<ST>(k.e_in, g) : 0.0;
-        h[c] = in ? ld<ST>(k.h_in, g) : 0.0;
+        /* the trailing pseudo cell is never owned, so never written back:
+         * its H is h[N] of the initial state at step 0 and zero after (the
+         * boundary condition) */
+        h[c] = (in && (g + 1 < (int64_t)k.G || n0 == 0))
+            ? ld<ST>(k.h_in, g)
+            : 0.0;
         /* no density: p keeps its initial zero (lindblad_noop) and is
          * neither read nor written */
         p[c] = (A::DC > 0 && in) ? ld<ST>(k.p_in, g) : 0.0;
@@ -455,7 +462,9 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 #pragma unroll
         for (int c = 0; c < C; c++) {
             const double ep = (c > 0) ? en[(c + C - 1) % C] : e_left;
-            if constexpr (A::FAST_FDTD) {
+            if constexpr (std::is_same<A, algo_noop_hz0>::value) {
+                h[c] = fma(fc[c], en[c] - ep, h[c]);
+            } else if constexpr (A::FAST_FDTD) {
                 h[c] = fma(fc[c], en[c] - ep, fb[c] * h[c]);
             } else {
                 h[c] = hz[c] ? 0.0 : h[c] + fc[c] * (en[c] - ep);

@@ -193,6 +193,30 @@ def test_deepest_temporal_block(gpu, case, form, tile):
                 assert rel_err(a, b) <= tol, (case, form, tb, rel_err(a, b))
 
 
+def test_noop_mixed_held_h_start(gpu):
+    """The FP32 stencil drops the factor that holds H at zero when every
+    such cell starts at zero (its coefficient is zero there, so H stays
+    zero); a state with H != 0 at those cells (h[0], h[N]) takes the kernel
+    with the factor. Both match the oracle."""
+    spec = setups.maxwell_line(num_gridpoints=2000, num_timesteps=600)
+    spec["records"] = [{"name": "e", "interval": spec["endtime"] / 8}]
+    dev, scen = setups.build(spec)
+    for hz in (0.0, 0.5):
+        prob = mb.build_problem(setups.solver_name(spec), dev, scen)
+        h = np.array(prob["h_init"], dtype=float)
+        h[0] = hz
+        h[-1] = hz
+        prob["h_init"] = h
+        print("case held h", hz, flush=True)
+        ctx = capi.Context([prob], capi.options("mixed", temporal_block=16))
+        ctx.run(0, int(prob["num_timesteps"]))
+        got = ctx.records(0)
+        want = oracle.OracleRun(prob).run().scratch[:len(got)]
+        assert np.max(np.abs(want)) > 0
+        assert rel_err(got, want) <= 1e-5, (hz, rel_err(got, want))
+        ctx.close()
+
+
 def test_state_after_partial_runs(gpu):
     """run_steps in pieces equals one call; fetch_state is the oracle's."""
     spec = setups.sit_2lvl(num_gridpoints=900, endtime=0.3e-12,
