@@ -142,7 +142,7 @@ __device__ __forceinline__ void rows_broadcast(double x, double* out)
 __device__ __forceinline__ double reduce_scatter4(const double* v)
 {
     double a0 = v[0], a2 = v[2], a1 = v[1], a3 = v[3];
-    swap32(a0, a2); /* a0 = [v0 v0' ...]: lower half v0, upper half v2 */
+    swap32(a0, a2); /* a0: lower halves of v0 and v2; a2: upper halves */
     swap32(a1, a3);
     double s0 = a0 + a2; /* rows 0, 1: v0 over the halves; 2, 3: v2 */
     double s1 = a1 + a3; /* rows 0, 1: v1; rows 2, 3: v3 */
