@@ -216,7 +216,10 @@ typedef struct mbh_problem {
     uint64_t zform_stride;
     const uint32_t* mat_idx; /* [num_gridpoints] */
     const double* e_init;    /* [num_gridpoints] */
-    const double* h_init;    /* [num_gridpoints + 1] */
+    const double* h_init;    /* [num_gridpoints + 1]; h_init[0] and
+                                h_init[num_gridpoints] are read by the first
+                                step only (the boundary condition holds H
+                                at zero there after every step) */
     const double* d_init;    /* [num_gridpoints][density stride] */
     uint32_t num_sources;
     const mbh_source* sources;
