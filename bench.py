@@ -1,34 +1,43 @@
 """Benchmark of the MI355X Maxwell-Bloch hot path.
 
-Workload (default, "v3"): the three-level V system on the Rodrigues
-propagator, 65536 cells over 150 um, run to 1.5 ps (392939 FDTD steps;
-mbsolve_amd.setups.v3_rodr).  A "step" is one FDTD time step of the whole
-grid: E update, sources, H update, density update of every cell, record
-copy.  Other workloads: "sit2" (two-level self-induced transparency,
-regularized Cayley, 8192 cells, 1.5 ps), "qcl6" (six-level gain medium,
-Rodrigues, 2^20 cells, the first 20000 of 4e6 steps), "sweep2" (512
-two-level runs of 2^12..2^17 cells, 49114 shared steps), "sweep6" (64
-six-level runs of 2^14..2^18 cells, the first 20000 of 4e6 steps),
-"maxwell" (the pure Maxwell stencil on 2^27 cells, 4000 steps).
+Workload (default, "sweep2"): 512 independent two-level self-induced
+transparency runs (regularized Cayley; grid lengths seeded log-uniform in
+2^12..2^17 cells, 19,039,104 cells in all; pulse areas U[0.5 pi, 4 pi]; T2
+log-uniform in 10 fs..1 ns; dx, dt and the 49114 time steps shared), FP32
+state storage, tile geometry from the autotuner (mbsolve_amd.setups.
+sit_2lvl_sweep).  One bench "step" is one pass: every run advanced from its
+initial state (mbh_reset) through all its time steps -- E update, sources,
+H update and boundary, density update of every cell, record copy -- i.e. the
+reference's solver run() (solver_cpu_fdtd.tpp:223-338) for all 512
+scenarios.  --warmup W passes are untimed, then --steps K passes are timed
+between device synchronisations.
 
-Candidate fast forms are timed on the same steps and their records compared
-with an FP64 reference run (relative error max|x - x_ref| / max|x_ref|, per
-record); the headline value is the fastest form within the 1e-5 bound.  The
-reference is the plain FP64 form (reference operation order) or, for the
-six-level workloads, the FP64 z-form (same arithmetic as the plain form up
-to rounding, 1e-13 from the oracle); the plain FP64 form is then timed on
---fp64-steps steps for its speed.  Device time comes from HIP events on the
-solver's stream; the wall clock brackets the same region after a device
-synchronisation.  A bounded sample of the CPU FP64 oracle (the restatement
-of the reference's cpu-fdtd solver) is timed on the host cores.
+Other workloads (--config): "sit2" (the single two-level SIT run, 8192
+cells, 49114 steps), "v3" (three-level V system on Rodrigues, 65536 cells,
+392939 steps), "qcl6" (six-level gain medium, 2^20 cells; a pass is the
+first 20000 of its 4e6 steps unless --pass-steps says otherwise), "sweep6"
+(64 six-level runs of 2^14..2^18 cells, passes as qcl6), "maxwell" (the pure
+Maxwell stencil on 2^27 cells, 4000 steps).
 
-    python bench.py [--steps K] [--warmup W]
-                    [--config v3|sit2|qcl6|sweep2|sweep6|maxwell]
-                    [--forms a,b] [--no-cpu] [--no-fp64]
+Before the timed passes every candidate form and the comparison forms run
+one pass each; their records are compared with those of a reference pass
+(max|x - x_ref| / max|x_ref| per record and run): the plain FP64 form (the
+reference's operation order) or, for the six-level workloads, the FP64
+z-form (1e-13 from the oracle; the plain form is then timed on a short pass
+for its speed).  The headline form is the fastest candidate within --tol.
+A few runs are also compared with the CPU FP64 oracle (the restatement of
+the reference's cpu-fdtd solver), where it finishes in seconds.  A bounded
+sample of that oracle is timed on the host cores as the CPU baseline.
+
+    python bench.py [--steps K] [--warmup W] [--config sweep2|sit2|v3|qcl6|
+                    sweep6|maxwell] [--forms a,b] [--pass-steps N]
+                    [--no-cpu] [--no-fp64] [--no-oracle]
 """
 import argparse
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -39,9 +48,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+MFMA_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA
 
-# algorithmic bytes per cell update (state read + write, no temporal reuse)
-# and algorithmic flops per cell update, per workload and storage precision
+# algorithmic bytes per cell update (state read + written once per step, no
+# temporal reuse) by workload and storage precision; algorithmic flops
 ALGO_BYTES = {("v3", "f32"): 80, ("v3", "f64"): 160,
               ("sit2", "f32"): 48, ("sit2", "f64"): 96,
               ("qcl6", "f32"): 296, ("qcl6", "f64"): 592,
@@ -50,27 +60,33 @@ ALGO_BYTES = {("v3", "f32"): 80, ("v3", "f64"): 160,
               ("maxwell", "f32"): 16, ("maxwell", "f64"): 32}
 ALGO_FLOPS = {"v3": 168, "sit2": 120, "qcl6": 2622, "sweep2": 120,
               "sweep6": 2622, "maxwell": 10}
-# per workload: default candidate forms, reference form for the errors,
-# default steps of the plain FP64 timing (0: same as the candidates)
-DEFAULTS = {"v3": ("mixed,zform,mfma,mfma64", "", 0),
-            "sit2": ("mixed,zform", "", 0),
-            "sweep2": ("mixed,zform", "", 0),
-            "qcl6": ("mfma,mfma64,zform", "zform", 400),
-            "sweep6": ("mfma,mfma64,zform", "zform", 400),
-            "maxwell": ("mixed", "", 0)}
-MFMA_PEAK_TFLOPS = 2500.0  # dense FP16 MFMA
-# v_mfma_f32_16x16x32_f16 per 16-cell block and step: split products (3, or
-# 4 with W - I in three pieces for three levels) x M-tiles x K chunks (six
-# levels: 3 x 3 x 2)
+# per workload: headline candidates, forms measured for comparison only,
+# reference form for the errors ("" = plain FP64), default pass length
+# (0 = the full run), steps of the plain FP64 timing when it is not the
+# reference, runs checked against the CPU oracle ("smallest:k" or "all")
+DEFAULTS = {
+    "sweep2": ("mixed", "zform", "", 0, 0, "smallest:4"),
+    "sit2": ("mixed,zform", "", "", 0, 0, "all"),
+    "v3": ("mixed,zform,mfma,mfma64", "", "", 0, 0, ""),
+    "qcl6": ("mfma,mfma64", "zform", "zform", 20000, 400, ""),
+    "sweep6": ("mfma,mfma64", "zform", "zform", 20000, 400, ""),
+    "maxwell": ("mixed", "", "", 0, 0, ""),
+}
+STORAGE = {"mixed": "f32", "mfma": "f32", "zform": "f64", "mfma64": "f64",
+           "": "f64"}
+# v_mfma_f32_16x16x32_f16 per 16-cell block and step (split products x
+# M-tiles x K chunks)
 MFMA_PER_BLOCK = {6: 18, 5: 6, 4: 3, 3: 4}
 MFMA_FLOPS = 2 * 16 * 16 * 32
+LEVELS = {"qcl6": 6, "sweep6": 6, "v3": 3, "sit2": 2, "sweep2": 2,
+          "maxwell": 0}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_specs(config):
+def make_specs(config, pass_steps):
     from mbsolve_amd import setups
     if config == "v3":
         return [setups.v3_rodr()], "3-level V system, Rodrigues, 65536 " \
@@ -79,82 +95,138 @@ def make_specs(config):
         return [setups.sit_2lvl()], "2-level SIT, regularized Cayley, " \
             "8192 cells, 1.5 ps (49114 steps)"
     if config == "qcl6":
-        return [setups.qcl6(num_timesteps=20000)], "6-level gain medium, " \
-            "Rodrigues, 2^20 cells, first 20000 of 4e6 steps"
+        return [setups.qcl6(num_timesteps=pass_steps)], \
+            "6-level gain medium, Rodrigues, 2^20 cells, first %d of 4e6 " \
+            "steps" % pass_steps
     if config == "sweep2":
         return setups.sit_2lvl_sweep(), "512 two-level SIT runs, " \
-            "2^12..2^17 cells, regularized Cayley, 49114 steps"
+            "2^12..2^17 cells (19,039,104 in all), regularized Cayley, " \
+            "FP32 storage, 49114 steps"
     if config == "maxwell":
         return [setups.maxwell_line()], "pure Maxwell FDTD (no density), " \
             "2^27 cells, 4000 steps"
     if config == "sweep6":
-        return setups.qcl6_sweep(num_timesteps=20000), "64 six-level " \
-            "gain-medium runs, 2^14..2^18 cells, Rodrigues, first 20000 " \
-            "of 4e6 steps"
+        return setups.qcl6_sweep(num_timesteps=pass_steps), "64 six-level " \
+            "gain-medium runs, 2^14..2^18 cells, Rodrigues, first %d of 4e6 " \
+            "steps" % pass_steps
     raise ValueError(config)
 
 
-def build_solver(specs, form, **opts):
-    import mbsolve_amd.lib as mb
-    from mbsolve_amd import setups
-    if len(specs) == 1:
-        dev, scen = setups.build(specs[0])
-        name = setups.solver_name(specs[0], form)
-        return mb.create_hip_solver(name, dev, scen, **opts)
-    runs = []
-    mb.material.clear_library()
-    for s in specs:
-        runs.append(setups.build(s, clear_library=False))
-    name = setups.solver_name(specs[0], form)
-    return mb.solver_hip_batch(name, runs, **opts)
+class Bench:
+    """One solver (single run or batch) of one form, driven pass by pass."""
+
+    def __init__(self, specs, form, opts):
+        import mbsolve_amd.lib as mb
+        from mbsolve_amd import setups
+        self.form = form
+        self.name = setups.solver_name(specs[0], form)
+        if len(specs) == 1:
+            dev, scen = setups.build(specs[0])
+            self.sol = mb.create_hip_solver(self.name, dev, scen, **opts)
+            self.batch = False
+            self.n_t = int(self.sol.problem()["num_timesteps"])
+        else:
+            mb.material.clear_library()
+            runs = [setups.build(s, clear_library=False) for s in specs]
+            self.sol = mb.solver_hip_batch(self.name, runs, **opts)
+            self.batch = True
+            self.n_t = int(self.sol.problem(0)["num_timesteps"])
+        self.info = self.sol.info()
+        self.cells = int(self.info["total_cells"])
+
+    def one_pass(self, steps):
+        """Rewinds to the initial state and runs steps [0, steps); returns
+        the device time (HIP events on the solver's stream around the
+        launches) in ms after waiting for it."""
+        self.sol.reset()
+        self.sol.run_steps(0, steps)
+        return self.sol.elapsed_ms()
+
+    def records(self):
+        """[[record arrays of run r] for r] after the last pass."""
+        self.sol.fetch_results()
+        if self.batch:
+            return [[x.get_data_real() for x in self.sol.get_results(r)]
+                    for r in range(self.sol.num_runs())]
+        return [[x.get_data_real() for x in self.sol.get_results()]]
+
+    def record_names(self):
+        res = self.sol.get_results(0) if self.batch else \
+            self.sol.get_results()
+        return [x.get_name() for x in res]
+
+    def problem(self, r):
+        return self.sol.problem(r) if self.batch else self.sol.problem()
+
+    def close(self):
+        self.sol = None
 
 
-def results_of(sol):
-    if hasattr(sol, "num_runs"):
-        out = []
-        for r in range(sol.num_runs()):
-            out.extend(x.get_data_real() for x in sol.get_results(r))
-        return out
-    return [x.get_data_real() for x in sol.get_results()]
+def rel_err(a, b):
+    scale = float(np.max(np.abs(b))) if b.size else 0.0
+    d = float(np.max(np.abs(a - b))) if b.size else 0.0
+    return d / scale if scale > 0 else d
 
 
-def timed_run(sol, warmup, steps, chunk_s=20.0):
-    """Warm-up steps [0, W), then the timed steps [W, W + K) in chunks of
-    about chunk_s seconds (one progress line each); returns the summed
-    device time of the chunks and the wall time of the timed region."""
-    if warmup:
-        sol.run_steps(0, warmup)
-        sol.fetch_results()  # waits for the device
-        w_ms = sol.elapsed_ms()
-        per_step = w_ms / warmup
-    else:
-        per_step = 0.0
-    chunk = steps if per_step <= 0 else \
-        max(1, min(steps, int(chunk_s * 1e3 / per_step)))
-    dev_ms = 0.0
+def error_table(recs, ref):
+    """[run][record] relative errors."""
+    return [[rel_err(a, b) for a, b in zip(ra, rb)]
+            for ra, rb in zip(recs, ref)]
+
+
+def summarize(errs, names):
+    """Per record name: max over runs, median over runs, run of the max."""
+    out = {}
+    E = np.asarray(errs, dtype=float)
+    for j, nm in enumerate(names):
+        col = E[:, j]
+        k = int(np.argmax(col))
+        out[nm] = {"max": float(col[k]), "median": float(np.median(col)),
+                   "worst_run": k}
+    out["all_max"] = float(E.max()) if E.size else 0.0
+    return out
+
+
+def timed_passes(b, steps, warmup, passes):
+    """`warmup` untimed passes, then `passes` timed ones between device
+    synchronisations (wall clock, and the summed device time of the
+    launches)."""
+    for i in range(warmup):
+        ms = b.one_pass(steps)
+        log("    warm-up pass %d/%d: %.1f ms device" % (i + 1, warmup, ms))
+    dev = 0.0
     t0 = time.perf_counter()
-    done = 0
-    while done < steps:
-        n = min(chunk, steps - done)
-        sol.run_steps(warmup + done, warmup + done + n)
-        dev_ms += sol.elapsed_ms()  # synchronises on the stop event
-        done += n
-        if done < steps:
-            log("    %d of %d steps, %.1f ms device" % (done, steps, dev_ms))
+    for i in range(passes):
+        ms = b.one_pass(steps)  # waits for the pass's stop event
+        dev += ms
+        log("    timed pass %d/%d: %.1f ms device" % (i + 1, passes, ms))
     wall = time.perf_counter() - t0
-    sol.fetch_results()
-    return dev_ms, wall
+    return wall, dev
 
 
-def _cpu_model():
+def _cpu_info():
+    model, phys = "unknown CPU", None
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE,
+                             stderr=subprocess.DEVNULL, text=True).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        model = kv.get("Model name", model)
+        cps, sockets = kv.get("Core(s) per socket"), kv.get("Socket(s)")
+        if cps and sockets and cps.isdigit() and sockets.isdigit():
+            phys = int(cps) * int(sockets)
     except OSError:
         pass
-    return "unknown CPU"
+    try:
+        nproc = int(subprocess.run(["nproc"], stdout=subprocess.PIPE,
+                                   text=True).stdout.strip())
+    except (OSError, ValueError):
+        nproc = os.cpu_count()
+    return {"model": model, "nproc": nproc, "physical_cores": phys,
+            "machine": platform.machine()}
 
 
 def _cpu_rate(prob, order, seconds, reps, threads):
@@ -184,21 +256,18 @@ def _cpu_rate(prob, order, seconds, reps, threads):
     return rates[len(rates) // 2], per, total, 2 * probe
 
 
-def cpu_baseline(specs, form_name, seconds=15.0, reps=5):
-    """Bounded sample of the CPU FP64 oracle on the same workload: the full
-    grid of run 0, both CPU time loops of the reference (plain
+def cpu_baseline(prob, seconds=15.0, reps=5):
+    """Bounded sample of the CPU FP64 oracle on one run of the workload
+    (its full grid): both CPU time loops of the reference (plain
     solver_cpu_fdtd and the redundant-halo solver_cpu_fdtd_red), each for
     `reps` timed pieces of as many steps as fit in about `seconds` / 2
-    together; the faster loop's median rate is reported.  OpenMP threads are
+    together; the faster loop's median rate is reported.  OpenMP threads:
+    OMP_NUM_THREADS (the host cores this GPU's share of the machine holds),
     pinned to cores (OMP_PROC_BIND=close, OMP_PLACES=cores, set in main
     before the oracle library loads)."""
-    import mbsolve_amd.lib as mb
-    from mbsolve_amd import setups
-    spec = specs[0]
-    dev, scen = setups.build(spec)
-    prob = mb.build_problem(setups.solver_name(spec), dev, scen)
+    info = _cpu_info()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
-        min(16, os.cpu_count() or 1)
+        (info["physical_cores"] or os.cpu_count() or 1)
     nx = int(prob["num_gridpoints"])
     res = {o: _cpu_rate(prob, o, seconds / 2, reps, threads)
            for o in ("plain", "red")}
@@ -207,41 +276,37 @@ def cpu_baseline(specs, form_name, seconds=15.0, reps=5):
     return {"value": rate, "unit": "cell-updates/s",
             "cores": threads, "kind": "port",
             "orders": {o: r[0] for o, r in res.items()},
-            "sample": "oracle (restated cpu-fdtd %s solver, FP64; the "
-                      "faster of the plain and red loops, plain %.3g, red "
-                      "%.3g cu/s) on the full %d-cell grid, median of %d "
-                      "pieces of %d steps after %d warm-up steps, %.1f s, "
-                      "%d OpenMP threads pinned to cores of %s"
-                      % ("red" if best == "red" else "plain",
-                         res["plain"][0], res["red"][0], nx, reps, per, warm,
-                         total, threads, _cpu_model())}
+            "host": info,
+            "sample": "oracle (restated cpu-fdtd solver, FP64, built -O3 "
+                      "-march=x86-64-v3; the faster of the plain (%.3g cu/s) "
+                      "and red (%.3g cu/s) loops) on one %d-cell run, median "
+                      "of %d pieces of %d steps after %d warm-up steps, %.1f "
+                      "s, %d OpenMP threads pinned to cores (nproc %s, %s "
+                      "physical cores, %s)"
+                      % (res["plain"][0], res["red"][0], nx, reps, per, warm,
+                         total, threads, info["nproc"],
+                         info["physical_cores"], info["model"])}
 
 
-def measure(specs, form, warmup, steps, opts):
-    """Builds the solver of `form`, runs warm-up + timed steps, returns
-    (value cu/s, device ms, wall s, info, records)."""
-    sol = build_solver(specs, form, **opts)
-    prob0 = sol.problem() if not hasattr(sol, "num_runs") else sol.problem(0)
-    n_t = int(prob0["num_timesteps"])
-    warmup = min(warmup, n_t - 1)
-    steps = min(steps if steps else n_t - warmup, n_t - warmup)
-    dev_ms, wall = timed_run(sol, warmup, steps)
-    info = sol.info()
-    cells = int(info["total_cells"])
-    recs = results_of(sol)
-    del sol
-    return {"value": cells * steps / (dev_ms * 1e-3), "dev_ms": dev_ms,
-            "wall": wall, "info": info, "records": recs, "steps": steps,
-            "warmup": warmup, "n_t": n_t, "cells": cells}
-
-
-def rel_errs(recs, ref):
-    errs = []
-    for a, b in zip(recs, ref):
-        scale = np.max(np.abs(b))
-        d = np.max(np.abs(a - b))
-        errs.append(float(d / scale) if scale > 0 else float(d))
-    return errs
+def oracle_check(bench_recs, ref_recs, probs, which, threads):
+    """Runs `which` through the CPU oracle; errors of the headline form and
+    of the reference form against it, per record name (max over runs)."""
+    import oracle
+    out = {"runs": [], "cells": [], "form": None, "reference": None}
+    ef, er = [], []
+    for r in which:
+        t0 = time.perf_counter()
+        orc = oracle.OracleRun(probs[r]).run(threads=threads).records()
+        log("    oracle run %d (%d cells): %.1f s" %
+            (r, int(probs[r]["num_gridpoints"]),
+             time.perf_counter() - t0))
+        ef.append([rel_err(a, b) for a, b in zip(bench_recs[r], orc)])
+        er.append([rel_err(a, b) for a, b in zip(ref_recs[r], orc)])
+        out["runs"].append(r)
+        out["cells"].append(int(probs[r]["num_gridpoints"]))
+    out["form"] = ef
+    out["reference"] = er
+    return out
 
 
 def copy_calibration(nbytes, reps=5):
@@ -267,29 +332,54 @@ def copy_calibration(nbytes, reps=5):
     hip.hipFree(b)
 
 
+def load_profile(config, form, info):
+    """Measured HBM traffic and VALU issue of the headline kernel at the
+    tuned geometry, from profiles/traffic_<config>_<form>.json (rocprofv3
+    FETCH_SIZE / WRITE_SIZE / SQ counters, see the file's command)."""
+    path = os.path.join(ROOT, "profiles", "traffic_%s_%s.json"
+                        % (config, form))
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    for g in t.get("geometries", [t]):
+        if (g.get("temporal_block") == info["temporal_block"] and
+                g.get("tile_cells") == info["tile_cells"]):
+            return g, os.path.relpath(path, ROOT)
+    return None, os.path.relpath(path, ROOT)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0,
-                    help="timed steps (default: the rest of the run)")
-    ap.add_argument("--warmup", type=int, default=2048)
-    ap.add_argument("--config", default="v3", choices=sorted(DEFAULTS))
+    ap.add_argument("--steps", type=int, default=3,
+                    help="timed passes (one pass = the whole workload "
+                         "from its initial state through all its steps)")
+    ap.add_argument("--warmup", type=int, default=1,
+                    help="untimed passes before the timed ones")
+    ap.add_argument("--config", default="sweep2", choices=sorted(DEFAULTS))
     ap.add_argument("--forms", default=None,
-                    help="candidate fast forms; the headline is the fastest "
-                         "one within the error bound of the FP64 reference")
+                    help="headline candidate forms (fastest within --tol)")
+    ap.add_argument("--compare", default=None,
+                    help="forms measured for comparison only")
     ap.add_argument("--ref", default=None,
                     help="reference form for the errors: '' (plain FP64) "
                          "or 'zform'")
+    ap.add_argument("--pass-steps", type=int, default=None,
+                    help="time steps per pass (default: the workload's)")
     ap.add_argument("--fp64-steps", type=int, default=None,
-                    help="steps of the plain FP64 timing when it is not "
-                         "the reference (default per workload)")
+                    help="pass length of the plain FP64 timing when it is "
+                         "not the reference")
     ap.add_argument("--tol", type=float, default=1e-5)
     ap.add_argument("--temporal-block", type=int, default=0)
     ap.add_argument("--tile-cells", type=int, default=0)
     ap.add_argument("--graph-launches", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-oracle", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--per-run-out", default=None,
+                    help="write the per-run error table (JSON) here")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     metavar="BYTES",
                     help="only run a device-to-device copy of BYTES")
@@ -312,138 +402,206 @@ def main():
         print("bench.py: no HIP device", file=sys.stderr)
         sys.exit(1)
 
-    d_forms, d_ref, d_fp64_steps = DEFAULTS[args.config]
-    forms = [f for f in (args.forms if args.forms is not None
-                         else d_forms).split(",") if f]
+    (d_cand, d_cmp, d_ref, d_pass, d_fp64_steps,
+     d_oracle) = DEFAULTS[args.config]
+    cands = [f for f in (args.forms if args.forms is not None
+                         else d_cand).split(",") if f]
+    compare = [f for f in (args.compare if args.compare is not None
+                           else d_cmp).split(",") if f and f not in cands]
     ref_form = args.ref if args.ref is not None else d_ref
+    pass_steps_req = args.pass_steps if args.pass_steps is not None \
+        else d_pass
     fp64_steps = (args.fp64_steps if args.fp64_steps is not None
                   else d_fp64_steps)
-    specs, workload = make_specs(args.config)
+    specs, workload = make_specs(args.config, pass_steps_req or 4000000)
     opts = dict(temporal_block=args.temporal_block,
                 tile_cells=args.tile_cells,
                 graph_launches=args.graph_launches)
+    cpu_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
+        min(16, os.cpu_count() or 1)
 
-    results = {}
-    for form in forms:
-        log("measuring form", form, "on", workload)
-        results[form] = measure(specs, form, args.warmup, args.steps, opts)
-        r = results[form]
-        log("  %s: %.3f ms device, %d launches, %.4g cu/s, info %s"
-            % (form, r["dev_ms"], r["info"]["launches"], r["value"],
-               r["info"]))
-    ref = None
-    fp64 = None
+    # reference pass: its records are what every form is compared with
+    ref_recs, ref_names, fp64, probs = None, None, None, None
     if not args.no_fp64:
-        if ref_form in results:
-            ref = results[ref_form]
-        else:
-            log("measuring the reference form '%s'" % ref_form)
-            ref = measure(specs, ref_form, args.warmup, args.steps, opts)
-            log("  ref: %.3f ms device, %.4g cu/s" % (ref["dev_ms"],
-                                                      ref["value"]))
-        for form, r in results.items():
-            r["errs"] = rel_errs(r["records"], ref["records"])
-            log("  %s max rel err vs the reference: %s"
-                % (form, ["%.3g" % x for x in r["errs"]]))
+        log("reference form '%s' on %s" % (ref_form or "fp64", workload))
+        b = Bench(specs, ref_form, opts)
+        n_pass = min(pass_steps_req or b.n_t, b.n_t)
+        ms = b.one_pass(n_pass)
+        ref_recs, ref_names = b.records(), b.record_names()
+        probs = [b.problem(r) for r in range(len(specs))]
+        log("  %.1f ms device, %.4g cu/s, info %s"
+            % (ms, b.cells * n_pass / (ms * 1e-3), b.info))
         if ref_form == "":
-            fp64 = ref
-        else:
-            log("timing the plain FP64 form on %s steps"
-                % (fp64_steps or "the same"))
-            fp64 = measure(specs, "", min(args.warmup, 64),
-                           fp64_steps or args.steps, opts)
-            log("  fp64: %.3f ms device, %.4g cu/s" % (fp64["dev_ms"],
-                                                       fp64["value"]))
+            fp64 = {"value": b.cells * n_pass / (ms * 1e-3), "steps": n_pass,
+                    "info": b.info}
+        b.close()
+        if ref_form != "":
+            log("plain FP64 form on a %d-step pass" % fp64_steps)
+            b = Bench(specs, "", opts)
+            ms = b.one_pass(fp64_steps)
+            fp64 = {"value": b.cells * fp64_steps / (ms * 1e-3),
+                    "steps": fp64_steps, "info": b.info}
+            log("  %.1f ms device, %.4g cu/s" % (ms, fp64["value"]))
+            b.close()
 
-    # headline: fastest form within the error bound (if measured)
-    ok = [f for f in forms
-          if ref is None or max(results[f]["errs"] or [0.0]) <= args.tol]
-    pool = ok if ok else forms
-    head = max(pool, key=lambda f: results[f]["value"])
-    r = results[head]
-    info = r["info"]
-    storage = "f32" if head in ("mixed", "mfma") else "f64"
-    launches = int(info["launches"])
-    per_launch_units = r["cells"] * info["temporal_block"]
-    launch_ms = r["dev_ms"] / max(launches, 1)
-    bpu = ALGO_BYTES.get((args.config, storage))
-    achieved = bpu * per_launch_units / (launch_ms * 1e-3) / 1e9
-    fpu = ALGO_FLOPS.get(args.config)
+    # one pass of every form: speed and errors
+    results, benches = {}, {}
+    for form in cands + compare:
+        log("form %s on %s" % (form, workload))
+        b = Bench(specs, form, opts)
+        n_pass = min(pass_steps_req or b.n_t, b.n_t)
+        ms = b.one_pass(n_pass)
+        r = {"value": b.cells * n_pass / (ms * 1e-3), "dev_ms": ms,
+             "info": b.info, "steps": n_pass, "solver": b.name}
+        if ref_recs is not None:
+            errs = error_table(b.records(), ref_recs)
+            r["errs"] = summarize(errs, ref_names)
+            r["per_run"] = errs
+        log("  %.1f ms device, %.4g cu/s, max rel err %s, info %s"
+            % (ms, r["value"], r.get("errs", {}).get("all_max"), b.info))
+        results[form] = r
+        if form in cands:
+            benches[form] = b
+        else:
+            b.close()
+
+    ok = [f for f in cands if "errs" not in results[f] or
+          results[f]["errs"]["all_max"] <= args.tol]
+    within_tol = bool(ok) if ref_recs is not None else None
+    head = max(ok or cands, key=lambda f: results[f]["value"])
+    for f in list(benches):
+        if f != head:
+            benches.pop(f).close()
+    b = benches[head]
+    n_pass = results[head]["steps"]
+    log("headline form %s: %d warm-up + %d timed passes of %d steps"
+        % (head, args.warmup, args.steps, n_pass))
+    wall, dev_ms = timed_passes(b, n_pass, args.warmup, args.steps)
+    cu_pass = b.cells * n_pass
+    value = cu_pass * args.steps / wall
+    info = b.info
+    sol_info = b.sol.info()
+    launches = int(sol_info["launches"])
+    # errors of the last timed pass (the headline's measured output)
+    head_recs = b.records()
+    errs = None
+    if ref_recs is not None:
+        per_run = error_table(head_recs, ref_recs)
+        errs = summarize(per_run, ref_names)
+        results[head]["per_run"] = per_run
+    orc = None
+    if ref_recs is not None and not args.no_oracle and d_oracle:
+        if d_oracle == "all":
+            which = list(range(len(specs)))
+        else:
+            k = int(d_oracle.split(":")[1])
+            which = sorted(range(len(specs)),
+                           key=lambda r: (int(probs[r]["num_gridpoints"]),
+                                          r))[:k]
+        log("CPU oracle on runs %s" % which)
+        orc = oracle_check(head_recs, ref_recs, probs, which, cpu_threads)
+        orc = {"runs": orc["runs"], "cells": orc["cells"],
+               "form": summarize(orc["form"], ref_names),
+               "reference": summarize(orc["reference"], ref_names)}
+    prob0 = b.problem(0)
+    b.close()
+
+    storage = STORAGE[head]
+    bpu = ALGO_BYTES[(args.config, storage)]
+    dev_pass_ms = dev_ms / args.steps
+    launch_ms = dev_pass_ms / max(launches, 1)
+    units_per_launch = cu_pass / max(launches, 1)
+    achieved = bpu * units_per_launch / (launch_ms * 1e-3) / 1e9
+    lev = LEVELS[args.config]
+    mfma_form = head.startswith("mfma")
     out = {
         "metric": "cell-updates/s, mixed vs FP64 GPU at <=1e-5 rel err; "
                   "% HBM/MFMA roofline",
-        "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1,
-        "steps": r["steps"], "warmup": r["warmup"],
-        "ms_per_step": r["dev_ms"] / r["steps"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "storage": storage, "data": "synthetic",
-        "config": {"workload": workload, "cells": r["cells"],
-                   "time_steps_total": r["n_t"], "form": head,
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 VALU + f16-split MFMA (f32 accumulate)" if mfma_form
+        else "f64", "storage": storage, "data": "synthetic",
+        "config": {"workload": workload, "runs": len(specs),
+                   "cells": b.cells, "time_steps": n_pass,
+                   "step": "one pass: every run from its initial state "
+                           "through %d time steps" % n_pass,
+                   "form": head, "solver": results[head]["solver"],
                    "temporal_block": info["temporal_block"],
                    "tile_cells": info["tile_cells"],
                    "owned_cells": info["owned_cells"],
-                   "launches": launches},
-        "wall_s": r["wall"],
+                   "launches_per_pass": launches,
+                   "graph_launches": info["graph_launches"]},
+        "device_ms_per_step": dev_pass_ms,
+        "within_tol": within_tol, "tol": args.tol,
+        "errors_vs": None if ref_recs is None else
+        ("plain FP64 GPU form" if ref_form == "" else
+         "FP64 z-form (GPU)"),
+        "errors": errs,
+        "oracle_errors": orc,
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_cell_update": bpu,
-                     "units_per_launch": per_launch_units,
+                     "units_per_launch": units_per_launch,
                      "launch_ms": launch_ms},
-        "compute": {"flops_per_cell_update": fpu,
-                    "achieved_tflops": fpu * r["value"] / 1e12,
-                    "fp64_peak_tflops": FP64_PEAK_TFLOPS,
-                    "frac": fpu * r["value"] / 1e12 / FP64_PEAK_TFLOPS},
-        "forms": {f: {"value": results[f]["value"],
-                      "max_rel_err_vs_ref": (max(results[f]["errs"])
-                                             if "errs" in results[f]
-                                             else None),
-                      "rel_err_per_record": results[f].get("errs"),
-                      "temporal_block":
-                          results[f]["info"]["temporal_block"],
-                      "tile_cells": results[f]["info"]["tile_cells"]}
-                  for f in forms},
-        "reference_form": ("fp64" if ref_form == "" else ref_form)
-        if ref is not None else None,
     }
-    if head.startswith("mfma"):
-        # matrix-core work of the tiles of one material with a density (the
-        # tiles that straddle runs, regions or the grid ends take the VALU
-        # z-form, vacuum tiles have no density)
-        lev = {"qcl6": 6, "sweep6": 6, "v3": 3}.get(args.config, 0)
+    fpu = ALGO_FLOPS[args.config]
+    out["compute"] = {"flops_per_cell_update": fpu,
+                      "achieved_tflops": fpu * value / 1e12,
+                      "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                      "frac": fpu * value / 1e12 / FP64_PEAK_TFLOPS}
+    prof, prof_path = load_profile(args.config, head, info)
+    if prof is not None:
+        out["roofline"]["traffic"] = prof.get("bytes_per_launch")
+        bind = {"source": prof_path}
+        if prof.get("bytes_per_launch"):
+            steps_l = prof.get("steps_per_launch", info["temporal_block"])
+            t_l = prof.get("kernel_ns", launch_ms * 1e6) * 1e-9
+            bind["measured_hbm_gbs"] = prof["bytes_per_launch"] / t_l / 1e9
+            bind["measured_hbm_frac"] = bind["measured_hbm_gbs"] / \
+                HBM_PEAK_GBS
+            bind["steps_per_launch"] = steps_l
+        if prof.get("valu_issue_frac") is not None:
+            bind["valu_issue_frac"] = prof["valu_issue_frac"]
+            bind["bound"] = "valu_issue"
+        if prof.get("mfma_busy_frac") is not None:
+            bind["mfma_busy_frac"] = prof["mfma_busy_frac"]
+        out["roofline"]["binding"] = bind
+    if mfma_form:
         blocks = 4 * info["mfma_tiles"] * launches * info["temporal_block"]
         hw = blocks * MFMA_PER_BLOCK.get(lev, 0) * MFMA_FLOPS
-        t = hw / (r["dev_ms"] * 1e-3) / 1e12
+        t = hw / (dev_pass_ms * 1e-3) / 1e12
         out["mfma"] = {"hardware_tflops": t, "peak_tflops": MFMA_PEAK_TFLOPS,
                        "frac": t / MFMA_PEAK_TFLOPS,
-                       "mfma_per_tile_step": 4 * MFMA_PER_BLOCK.get(lev, 0),
-                       "instr": "v_mfma_f32_16x16x32_f16, split FP16 "
-                                "products (W - I in %d pieces)"
-                                % (3 if lev <= 3 else 2)}
-    traffic_file = os.path.join(ROOT, "profiles", "traffic_%s_%s.json"
-                                % (args.config, head))
-    if os.path.exists(traffic_file):
-        with open(traffic_file) as f:
-            t = json.load(f)
-        # one measured geometry, or a list of them under "geometries"
-        for g in t.get("geometries", [t]):
-            if (g.get("temporal_block") == info["temporal_block"] and
-                    g.get("tile_cells") == info["tile_cells"]):
-                out["roofline"]["traffic"] = g.get("bytes_per_launch")
-    # the figure mbsolve-tool prints (mbsolve-tool.cpp:693-695): N_x T / dt
-    # per run time, i.e. n_t - 1 counted steps per n_t run steps, here for
-    # a full run at the measured per-step rate
-    out["tool_rate"] = r["value"] * (r["n_t"] - 1) / r["n_t"]
+                       "instr": "v_mfma_f32_16x16x32_f16"}
+    out["tool_rate"] = value * (n_pass - 1) / n_pass
     if fp64 is not None:
         out["fp64_value"] = fp64["value"]
         out["fp64_steps"] = fp64["steps"]
-        out["speedup_vs_fp64"] = r["value"] / fp64["value"]
-    if ref is not None:
-        out["max_rel_err_vs_ref"] = max(r["errs"]) if r["errs"] else 0.0
+        out["speedup_vs_fp64"] = value / fp64["value"]
+    fastest64 = [f for f in results if STORAGE[f] == "f64"]
+    if fastest64:
+        f64 = max(fastest64, key=lambda f: results[f]["value"])
+        out["speedup_vs_fastest_fp64_form"] = {
+            "form": f64, "value": results[f64]["value"],
+            "speedup": value / results[f64]["value"]}
+    out["forms"] = {f: {"value": r["value"], "storage": STORAGE[f],
+                        "max_rel_err": (r["errs"]["all_max"]
+                                        if "errs" in r else None),
+                        "temporal_block": r["info"]["temporal_block"],
+                        "tile_cells": r["info"]["tile_cells"],
+                        "graph_launches": r["info"]["graph_launches"]}
+                    for f, r in results.items()}
+    if args.per_run_out:
+        with open(args.per_run_out, "w") as f:
+            json.dump({"records": ref_names,
+                       "forms": {k: r.get("per_run")
+                                 for k, r in results.items()}}, f)
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(specs, head,
-                                           seconds=args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(prob0, seconds=args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

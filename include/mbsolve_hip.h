@@ -235,8 +235,14 @@ typedef struct mbh_options {
                                 at most 64 (a deeper request is clamped) */
     uint32_t tile_cells;     /* cells per wavefront tile incl. halo, 0 = auto */
     int32_t device;          /* HIP device ordinal, -1 = current */
-    uint32_t graph_launches; /* kernel launches per captured hipGraph,
-                                0 = auto, 1 = no graph */
+    uint32_t graph_launches; /* kernel launches per captured hipGraph
+                                (rounded up to an even count), 1 = no
+                                graph, 0 = auto: graphs of about 2 ms of
+                                launches when one launch of the tuned
+                                geometry takes at most 0.25 ms and a full
+                                run replays the graph at least twice, no
+                                graph otherwise (mbh_info.graph_launches
+                                reports the choice) */
     uint32_t step_order;     /* MBH_ORDER_* */
     uint32_t reserved[2];
 } mbh_options;
@@ -254,8 +260,12 @@ typedef struct mbh_info {
     uint32_t step_order;   /* MBH_ORDER_* */
     uint32_t mfma_tiles;   /* MFMA forms: tiles whose density update runs on
                               the matrix cores (one material with a density) */
-    uint32_t reserved[1];
+    uint32_t flags;        /* MBH_INFO_* */
 } mbh_info;
+
+/* mbh_info.flags: the pure-Maxwell FP32 kernel without the H factor of
+ * the cells held at zero is in use (every such cell starts with H = 0) */
+#define MBH_INFO_HZ_ZERO 1u
 
 typedef struct mbh_ctx mbh_ctx;
 
@@ -283,6 +293,14 @@ int mbh_create(const mbh_problem* runs,
  * solver_cpu_fdtd_red.tpp:351-567).  Returns once the work is queued on the
  * context's stream; use mbh_synchronize or a fetch to wait. */
 int mbh_run(mbh_ctx* ctx, uint64_t n_begin, uint64_t n_end);
+
+/* Rewinds every run to the state handed to mbh_create (the state before
+ * step 0) and zeroes the record scratch, so that mbh_run(ctx, 0, n_t)
+ * repeats the simulation.  Queued on the context's stream.  No counterpart
+ * in the reference, whose solver runs once per construction
+ * (solver_cpu_fdtd.tpp:223-338): for repeated sweeps and benchmarks that
+ * keep one context. */
+int mbh_reset(mbh_ctx* ctx);
 
 /* Waits for all queued work of the context. */
 int mbh_synchronize(mbh_ctx* ctx);

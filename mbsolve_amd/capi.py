@@ -26,7 +26,8 @@ FORMS = {"fp64": MBH_FORM_FP64, "mixed": MBH_FORM_MIXED,
          "mfma64": MBH_FORM_MFMA64}
 
 EXPORTED = ["mbh_version", "mbh_device_count", "mbh_default_options",
-            "mbh_create", "mbh_run", "mbh_synchronize", "mbh_elapsed_ms",
+            "mbh_create", "mbh_run", "mbh_reset", "mbh_synchronize",
+            "mbh_elapsed_ms",
             "mbh_fetch_records", "mbh_fetch_state", "mbh_get_info",
             "mbh_destroy", "mbh_last_error"]
 
@@ -76,7 +77,9 @@ class mbh_info(C.Structure):
                 ("num_tiles", C.c_uint64), ("total_cells", C.c_uint64),
                 ("launches", C.c_uint64), ("state_bytes", C.c_uint64),
                 ("graph_launches", C.c_uint32), ("step_order", C.c_uint32),
-                ("mfma_tiles", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("mfma_tiles", C.c_uint32), ("flags", C.c_uint32)]
+
+MBH_INFO_HZ_ZERO = 1
 
 
 _lib = None
@@ -103,6 +106,8 @@ def load():
                                C.POINTER(mbh_options), C.POINTER(vp)]
     lib.mbh_run.restype = C.c_int
     lib.mbh_run.argtypes = [vp, C.c_uint64, C.c_uint64]
+    lib.mbh_reset.restype = C.c_int
+    lib.mbh_reset.argtypes = [vp]
     lib.mbh_synchronize.restype = C.c_int
     lib.mbh_synchronize.argtypes = [vp]
     lib.mbh_elapsed_ms.restype = C.c_int
@@ -227,6 +232,9 @@ class Context:
 
     def run(self, n_begin, n_end):
         self._check(self.lib.mbh_run(self.ctx, n_begin, n_end))
+
+    def reset(self):
+        self._check(self.lib.mbh_reset(self.ctx))
 
     def synchronize(self):
         self._check(self.lib.mbh_synchronize(self.ctx))

@@ -59,6 +59,10 @@ public:
     /** Queues steps [n_begin, n_end) without copying records back. */
     void run_steps(uint64_t n_begin, uint64_t n_end) const;
 
+    /** Rewinds to the initial state and clears the record scratch (queued;
+     * the next run() repeats the simulation). */
+    void reset() const;
+
     /** Waits for the device and copies the record scratch into the
      * results. */
     void fetch_results() const;
@@ -111,6 +115,9 @@ public:
     void run() const;
 
     void run_steps(uint64_t n_begin, uint64_t n_end) const;
+
+    /** Rewinds all runs to their initial state (see solver_hip_fdtd). */
+    void reset() const;
 
     void fetch_results() const;
 

@@ -119,6 +119,7 @@ py::dict info_dict(const mbh_info& i)
     d["graph_launches"] = i.graph_launches;
     d["step_order"] = i.step_order;
     d["mfma_tiles"] = i.mfma_tiles;
+    d["flags"] = i.flags;
     return d;
 }
 
@@ -417,6 +418,8 @@ PYBIND11_MODULE(_mbsolve, m)
                  py::gil_scoped_release g;
                  as_hip(s)->fetch_results();
              })
+        .def("reset",
+             [](const std::shared_ptr<solver>& s) { as_hip(s)->reset(); })
         .def("elapsed_ms",
              [](const std::shared_ptr<solver>& s) {
                  return as_hip(s)->elapsed_ms();
@@ -493,6 +496,7 @@ PYBIND11_MODULE(_mbsolve, m)
              py::call_guard<py::gil_scoped_release>())
         .def("fetch_results", &solver_hip_batch::fetch_results,
              py::call_guard<py::gil_scoped_release>())
+        .def("reset", &solver_hip_batch::reset)
         .def("elapsed_ms", &solver_hip_batch::elapsed_ms)
         .def("info",
              [](const solver_hip_batch& b) { return info_dict(b.info()); })

@@ -165,6 +165,12 @@ solver_hip_fdtd::run_steps(uint64_t n_begin, uint64_t n_end) const
 }
 
 void
+solver_hip_fdtd::reset() const
+{
+    check(mbh_reset(m_ctx), m_ctx, "mbh_reset");
+}
+
+void
 solver_hip_fdtd::fetch_results() const
 {
     copy_results(m_ctx, 0, m_problem);
@@ -240,6 +246,12 @@ void
 solver_hip_batch::run_steps(uint64_t n_begin, uint64_t n_end) const
 {
     check(mbh_run(m_ctx, n_begin, n_end), m_ctx, "mbh_run");
+}
+
+void
+solver_hip_batch::reset() const
+{
+    check(mbh_reset(m_ctx), m_ctx, "mbh_reset");
 }
 
 void

@@ -28,6 +28,24 @@ thread_local std::string g_last_error;
 __global__ void set_u64(uint64_t* p, uint64_t v) { *p = v; }
 __global__ void add_u64(uint64_t* p, uint64_t v) { *p += v; }
 
+/* H of the trailing pseudo cell (h[N] of the last run) in both state
+ * buffers, once step 0 has consumed it */
+template<typename ST>
+__global__ void zero_last_h(ST* h0, ST* h1, uint64_t g)
+{
+    h0[g] = ST(0);
+    h1[g] = ST(0);
+}
+
+__global__ void fill_zero_f64(double* p, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        p[i] = 0.0;
+    }
+}
+
 struct dev_buf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -355,6 +373,8 @@ struct mbh_ctx {
     /* device memory */
     std::vector<void*> allocs;
     void* state[2][4] = { { nullptr } }; /* [buffer][e,h,p,d] */
+    void* init[4] = { nullptr };         /* initial state (mbh_reset) */
+    uint64_t state_elems[4] = { 0 };     /* elements per state array */
     int cur = 0;
     int32_t* mat = nullptr;
     uint8_t* hzero = nullptr;
@@ -380,7 +400,8 @@ struct mbh_ctx {
     uint32_t* tile_rec_ptr = nullptr;
     uint32_t* tile_rec_idx = nullptr;
     double* src_data = nullptr;
-    double* scratch = nullptr;
+    double* rec_buf = nullptr; /* record scratch of all runs */
+    uint64_t rec_total = 0;    /* doubles in rec_buf */
     uint64_t* n_base = nullptr;
 
     /* host copies for rebuilding per-geometry tile tables */
@@ -391,6 +412,8 @@ struct mbh_ctx {
     /* graphs */
     uint32_t graph_launches = 1;
     hipGraphExec_t graph[2] = { nullptr, nullptr }; /* per start buffer */
+    double launch_ms_est = 0.0; /* one launch of the chosen geometry */
+    uint32_t kform = 0;         /* kernel table form in use */
 
     uint64_t last_launches = 0;
     uint64_t state_bytes = 0;
@@ -792,6 +815,7 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
         ctx->hz_zero) {
         kform = KFORM_NOOP_HZ0;
     }
+    ctx->kform = kform;
     std::vector<uint32_t> cs;
     for (uint32_t C : { 1u, 2u, 4u, 8u }) {
         if (find_kernel(ctx->algo, ctx->levels, kform, C, ctx->order) &&
@@ -827,15 +851,10 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
         cand.emplace_back(C, std::min<uint32_t>({req_tb, 32u * C - 4u,
                                                  (uint32_t)MAX_BLOCK}));
     }
-    if (cand.size() == 1) {
-        set_geometry(ctx, cand[0].first, cand[0].second);
-        ctx->launch =
-            find_kernel(ctx->algo, ctx->levels, kform, ctx->C, ctx->order)
-                ->fn;
-        return MBH_OK;
-    }
     /* cheapest candidates (shallow blocks) first would bias the budget;
-     * the list is tried in order until ~0.5 s of trial launches are spent */
+     * the list is tried in order until ~0.5 s of trial launches are spent
+     * (a single candidate is timed too: its launch time sets the automatic
+     * graph size) */
     double best = 1e300, spent_ms = 0.0;
     std::pair<uint32_t, uint32_t> pick = cand[0];
     size_t tried = 0;
@@ -888,6 +907,7 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
         }
     }
     ctx->tuned_candidates = tried;
+    ctx->launch_ms_est = best * pick.second;
     set_geometry(ctx, pick.first, pick.second);
     ctx->launch =
         find_kernel(ctx->algo, ctx->levels, kform, ctx->C, ctx->order)->fn;
@@ -939,6 +959,31 @@ hipError_t launch_block(const mbh_ctx* ctx, KArgs k)
     k.tile_mat = ctx->split ? ctx->tiles_mat : nullptr;
     k.n_mixed = ctx->split ? (uint32_t)ctx->n_tiles_mixed : 0u;
     return ctx->launch(k, ctx->num_tiles, ctx->stream);
+}
+
+/*
+ * Launches per captured graph for graph_launches == 0.  A graph pays off
+ * only where the gaps between launches are a visible share of the time:
+ * launches shorter than AUTO_GRAPH_MAX_LAUNCH_MS are grouped into graphs of
+ * an even count covering about AUTO_GRAPH_SPAN_MS, provided a full run
+ * replays such a graph at least twice; longer launches go one by one.
+ */
+constexpr double AUTO_GRAPH_MAX_LAUNCH_MS = 0.25;
+constexpr double AUTO_GRAPH_SPAN_MS = 2.0;
+constexpr uint32_t AUTO_GRAPH_MAX = 128;
+
+uint32_t auto_graph_launches(const mbh_ctx* ctx)
+{
+    const double t = ctx->launch_ms_est;
+    if (!(t > 0.0) || t > AUTO_GRAPH_MAX_LAUNCH_MS) {
+        return 1u;
+    }
+    uint32_t L = 2u * (uint32_t)std::ceil(AUTO_GRAPH_SPAN_MS / (2.0 * t));
+    L = std::min(std::max(L, 2u), AUTO_GRAPH_MAX);
+    while (L >= 2u && 2ull * L * ctx->tb > ctx->n_t) {
+        L -= 2u;
+    }
+    return L >= 2u ? L : 1u;
 }
 
 int build_graph(mbh_ctx* ctx, int start_buf)
@@ -1142,6 +1187,7 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
      * parity is the same before and after a replay */
     c->graph_launches =
         (opt.graph_launches > 1) ? ((opt.graph_launches + 1u) & ~1u) : 1u;
+    const bool graph_auto = (opt.graph_launches == 0);
     if (!find_kernel(c->algo, c->levels,
                      kernel_form(c->algo, c->levels, c->form), 1, c->order)) {
         return fail(nullptr, MBH_ERR_UNSUPPORTED,
@@ -1287,6 +1333,16 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
         c->cur = 0;
     }
 
+    /* a copy of the initial state for mbh_reset */
+    c->state_elems[0] = c->state_elems[1] = c->state_elems[2] = G;
+    c->state_elems[3] = std::max<uint64_t>(1, c->DC) * c->dstride;
+    for (int j = 0; j < 4; j++) {
+        RC(dalloc(c, &c->init[j], c->state_elems[j] * esz));
+        HIP_TRY(c, hipMemcpy(c->init[j], c->state[0][j],
+                             c->state_elems[j] * esz,
+                             hipMemcpyDeviceToDevice));
+    }
+
     /* source tables, sorted by cell; stable, so sources at one cell keep
      * the reference's list order */
     {
@@ -1320,9 +1376,10 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
 
     /* record scratch and descriptors */
     {
-        RC(dalloc(c, (void**)&c->scratch,
+        c->rec_total = std::max<uint64_t>(1, scratch_total);
+        RC(dalloc(c, (void**)&c->rec_buf,
                   std::max<uint64_t>(1, scratch_total) * sizeof(double)));
-        HIP_TRY(c, hipMemset(c->scratch, 0,
+        HIP_TRY(c, hipMemset(c->rec_buf, 0,
                              std::max<uint64_t>(1, scratch_total) *
                                  sizeof(double)));
         for (uint32_t r = 0; r < num_runs; r++) {
@@ -1335,7 +1392,7 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
                 d.cols = R.cols;
                 d.rows = R.rows;
                 d.ratio = R.interval_ratio;
-                d.dst = c->scratch + L.scratch_off + R.offset_real;
+                d.dst = c->rec_buf + L.scratch_off + R.offset_real;
                 d.type = R.type;
                 d.row_idx = R.row_idx;
                 d.col_idx = R.col_idx;
@@ -1363,6 +1420,9 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     /* launch geometry: measured, then the final per-tile tables */
     RC(autotune(c, opt.temporal_block, opt.tile_cells));
     RC(build_tile_tables(c, true));
+    if (graph_auto) {
+        c->graph_launches = auto_graph_launches(c);
+    }
     HIP_TRY(c, hipDeviceSynchronize());
 #undef RC
     *out = ctx.release();
@@ -1413,9 +1473,44 @@ int mbh_run(mbh_ctx* ctx, uint64_t n_begin, uint64_t n_end)
         n += steps;
         launches++;
     }
+    if (n_begin == 0 && n_end > 0) {
+        /* step 0 has read h[N] of the last run (the trailing pseudo cell,
+         * never written back by a tile); the boundary condition holds it
+         * at zero from then on */
+        const uint64_t g = ctx->G - 1;
+        if (ctx->fp32) {
+            hipLaunchKernelGGL(zero_last_h<float>, dim3(1), dim3(1), 0,
+                               ctx->stream, (float*)ctx->state[0][1],
+                               (float*)ctx->state[1][1], g);
+        } else {
+            hipLaunchKernelGGL(zero_last_h<double>, dim3(1), dim3(1), 0,
+                               ctx->stream, (double*)ctx->state[0][1],
+                               (double*)ctx->state[1][1], g);
+        }
+        HIP_TRY(ctx, hipGetLastError());
+    }
     HIP_TRY(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timed = true;
     ctx->last_launches = launches;
+    return MBH_OK;
+}
+
+int mbh_reset(mbh_ctx* ctx)
+{
+    if (!ctx) {
+        return fail(nullptr, MBH_ERR_INVALID, "ctx is NULL");
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    const size_t esz = ctx->fp32 ? sizeof(float) : sizeof(double);
+    for (int j = 0; j < 4; j++) {
+        HIP_TRY(ctx, hipMemcpyAsync(ctx->state[0][j], ctx->init[j],
+                                    ctx->state_elems[j] * esz,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    hipLaunchKernelGGL(fill_zero_f64, dim3(256), dim3(256), 0, ctx->stream,
+                       ctx->rec_buf, ctx->rec_total);
+    HIP_TRY(ctx, hipGetLastError());
+    ctx->cur = 0;
     return MBH_OK;
 }
 
@@ -1456,7 +1551,7 @@ int mbh_fetch_records(mbh_ctx* ctx, uint32_t run, double* dst,
     }
     HIP_TRY(ctx, hipSetDevice(ctx->device));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-    HIP_TRY(ctx, hipMemcpy(dst, ctx->scratch + ctx->runs[run].scratch_off,
+    HIP_TRY(ctx, hipMemcpy(dst, ctx->rec_buf + ctx->runs[run].scratch_off,
                            count * sizeof(double), hipMemcpyDeviceToHost));
     return MBH_OK;
 }
@@ -1588,6 +1683,7 @@ int mbh_get_info(const mbh_ctx* ctx, mbh_info* info)
     info->graph_launches = ctx->graph_launches;
     info->step_order = ctx->order;
     info->mfma_tiles = (uint32_t)ctx->n_tiles_mfma;
+    info->flags = (ctx->kform == KFORM_NOOP_HZ0) ? MBH_INFO_HZ_ZERO : 0u;
     return MBH_OK;
 }
 

@@ -209,11 +209,25 @@ def test_noop_mixed_held_h_start(gpu):
         prob["h_init"] = h
         print("case held h", hz, flush=True)
         ctx = capi.Context([prob], capi.options("mixed", temporal_block=16))
+        # the kernel without the H factor runs exactly when every held
+        # cell starts at zero
+        flags = ctx.info()["flags"]
+        assert bool(flags & capi.MBH_INFO_HZ_ZERO) == (hz == 0.0), (hz, flags)
         ctx.run(0, int(prob["num_timesteps"]))
         got = ctx.records(0)
-        want = oracle.OracleRun(prob).run().scratch[:len(got)]
+        orc = oracle.OracleRun(prob).run()
+        want = orc.scratch[:len(got)]
         assert np.max(np.abs(want)) > 0
         assert rel_err(got, want) <= 1e-5, (hz, rel_err(got, want))
+        # h[0] and h[N] are held at zero once step 0 has run, also in the
+        # fetched state (the trailing pseudo cell is never written back by
+        # a tile)
+        st = ctx.state(0)
+        ref = orc.state()
+        assert st["h"][0] == 0.0 and st["h"][-1] == 0.0, hz
+        assert ref["h"][0] == 0.0 and ref["h"][-1] == 0.0, hz
+        assert rel_err(st["e"], ref["e"]) <= 1e-5, hz
+        assert rel_err(st["h"], ref["h"]) <= 1e-5, hz
         ctx.close()
 
 
@@ -341,3 +355,78 @@ def test_red_order_batch_and_partial_runs(gpu):
     for r, ref in enumerate(singles):
         for x, b in zip(batch.get_results(r), ref):
             assert np.array_equal(x.get_data_real(), b), r
+
+
+@pytest.mark.parametrize("name,form", [
+    ("hip-fdtd-2lvl-reg-cayley", "fp64"),
+    ("hip-fdtd-2lvl-reg-cayley-mixed", "mixed"),
+    ("hip-fdtd-3lvl-rodr-mfma", "mfma"),
+    ("hip-fdtd-noop-mixed", "mixed"),
+])
+def test_reset_repeats_the_run(gpu, name, form):
+    """mbh_reset rewinds to the initial state and clears the records: a
+    second full run after it reproduces the first bit for bit (records and
+    final state), also with a nonzero initial h[N] (read again at step 0
+    of the repeated run)."""
+    if "noop" in name:
+        spec = setups.maxwell_line(num_gridpoints=3000, num_timesteps=700)
+        spec["records"] = [{"name": "e", "interval": spec["endtime"] / 8}]
+    elif "rodr" in name:
+        spec = setups.v3_rodr(num_gridpoints=1500, endtime=0.3e-12,
+                              num_records=8)
+    else:
+        spec = setups.sit_2lvl(num_gridpoints=1500, endtime=0.3e-12,
+                               num_records=8)
+    dev, scen = setups.build(spec)
+    prob = mb.build_problem(name, dev, scen)
+    if "noop" in name:
+        h = np.array(prob["h_init"], dtype=float)
+        h[-1] = 0.25
+        prob["h_init"] = h
+    print("case reset", name, flush=True)
+    ctx = capi.Context([prob], capi.options(form))
+    nt = int(prob["num_timesteps"])
+    ctx.run(0, nt)
+    rec1, st1 = ctx.records(0), ctx.state(0)
+    assert np.max(np.abs(rec1)) > 0
+    # the same launches again (FP32 storage rounds the state at every
+    # launch boundary, so only the FP64 form is also independent of where
+    # a run is split)
+    pieces = [(0, nt // 3), (nt // 3, nt)] if form == "fp64" else [(0, nt)]
+    for _ in range(2):
+        ctx.reset()
+        for a, b in pieces:
+            ctx.run(a, b)
+        rec2, st2 = ctx.records(0), ctx.state(0)
+        assert np.array_equal(rec1, rec2)
+        for k in ("e", "h", "p", "d"):
+            assert np.array_equal(st1[k], st2[k]), k
+    ctx.close()
+
+
+def test_graph_launches_auto(gpu):
+    """graph_launches = 0 groups the short launches of a small grid into
+    captured graphs (even count); the records and the state equal those of
+    launches one by one."""
+    spec = setups.sit_2lvl(num_gridpoints=1200, endtime=0.6e-12,
+                           num_records=16)
+    out = []
+    for graph in (1, 0):
+        dev, scen = setups.build(spec)
+        sol = mb.create_hip_solver("hip-fdtd-2lvl-reg-cayley-zform", dev,
+                                   scen, graph_launches=graph)
+        info = sol.info()
+        print("case graph", graph, info, flush=True)
+        if graph == 0:
+            assert info["graph_launches"] >= 2
+            assert info["graph_launches"] % 2 == 0
+        sol.run()
+        out.append(([r.get_data_real() for r in sol.get_results()],
+                    sol.fetch_state(), info))
+    # per-cell arithmetic does not depend on the tile geometry the
+    # autotuner picks for either context
+    (r1, s1, _), (r0, s0, _) = out
+    for a, b in zip(r1, r0):
+        assert np.array_equal(a, b)
+    for k in ("e", "h", "p", "d"):
+        assert np.array_equal(s1[k], s0[k]), k
