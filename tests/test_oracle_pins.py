@@ -1,0 +1,341 @@
+"""The oracle's remaining functions against what the mathematics fixes.
+
+* population readout (coherence_vector_representation.hpp:289-336) against
+  rho = I/N + 1/2 sum d_i g_i for random Hermitian trace-1 matrices;
+* the FDTD coefficients (common_fdtd.hpp:158-190) through the stencil:
+  uniform decay of E in a lossy medium, the curl terms of a linear ramp,
+  and the Crank-Nicolson closed form of the loss;
+* the Yee stencil and its magnetic walls (solver_cpu_fdtd.tpp:233-275):
+  the discrete dispersion relation of standing modes and the sign of the
+  field reflected at a wall;
+* the record rows (copy_list_entry.hpp:96-108,135-145) and what a row
+  holds (the state after its step, solver_cpu_fdtd.tpp:287-325);
+* the regularized Cayley step (algo_lindblad_reg_cayley.hpp:219-257)
+  without relaxation against U rho U^H with the Cayley factor built here;
+* the marskar2011 six-level set-up (mbsolve-tool.cpp:479-662) against the
+  digitized rho55 / rho66 curves of tools/reference-data.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from mbsolve_amd import setups
+from oracle import qm_setup
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+C0 = 1.0 / math.sqrt(qm_setup.MU0 * qm_setup.EPS0)
+
+
+# ---------------------------------------------------------- populations ----
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 6])
+def test_population_readout_against_definition(n):
+    """orc_cv_population(d, m) = Re rho_mm for d_i = tr(rho g_i), with
+    general (coherent) Hermitian trace-1 rho."""
+    lib = oracle.load()
+    g = qm_setup.generators(n)
+    rng = np.random.default_rng(100 + n)
+    for _ in range(5):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        rho = a @ a.conj().T
+        rho /= np.trace(rho).real
+        d = np.array([np.trace(rho @ gi).real for gi in g])
+        # the expansion the readout inverts
+        assert np.abs(np.eye(n) / n + 0.5 * sum(
+            di * gi for di, gi in zip(d, g)) - rho).max() < 1e-14
+        dd = np.ascontiguousarray(d)
+        got = [lib.orc_cv_population(n, oracle._ptr(dd), m)
+               for m in range(n)]
+        want = np.diag(rho).real
+        assert np.abs(np.array(got) - want).max() < 1e-14, (n, got, want)
+        assert abs(sum(got) - 1.0) < 1e-14
+
+
+# --------------------------------------------------- FDTD coefficients -----
+def _line(nx, material, endtime_steps=40):
+    """A one-material line with dx = 2^-22 m (the cell positions i dx are
+    exact, so every cell lies inside the region)."""
+    dx = 2.0 ** -22
+    length = (nx - 1) * dx
+    spec = {"name": "line", "algo": "noop", "num_levels": 0,
+            "materials": {"M": material},
+            "regions": [("r", "M", 0.0, length)],
+            "num_gridpoints": nx, "endtime": 1.0,
+            "rho_init": ([1.0, 0.0], [])}
+    eps = material.get("rel_permittivity", 1.0)
+    dt = 0.5 * dx * math.sqrt(eps) / C0
+    spec["endtime"] = dt * endtime_steps
+    return qm_setup.build_problem(spec)
+
+
+def test_lossy_uniform_field_decay():
+    """Uniform E, zero H in a lossy dielectric: H stays zero and every E
+    value is multiplied by fac_a per step, exactly; fac_a^n follows the
+    continuous decay exp(-sigma t / eps) to the Crank-Nicolson error
+    (1 - x)/(1 + x) - exp(-2x) = -(2/3) x^3 + O(x^4) per step
+    (x = sigma dt / 2 eps), with sigma = 2 alpha sqrt(eps/mu) for the
+    amplitude attenuation alpha = losses."""
+    alpha, eps_r = 2.0e4, 2.5
+    prob = _line(300, {"rel_permittivity": eps_r, "losses": alpha})
+    fa, fb, fc, gam = prob["fdtd_consts"][:4]
+    dt, dx = prob["dt"], prob["dx"]
+    eps = qm_setup.EPS0 * eps_r
+    sigma = 2.0 * alpha * math.sqrt(eps / qm_setup.MU0)
+    x = sigma * dt / (2.0 * eps)
+    assert fa == pytest.approx((1 - x) / (1 + x), rel=1e-15)
+    assert fb == pytest.approx(dt / eps / (1 + x), rel=1e-15)
+    assert fc == pytest.approx(dt / (qm_setup.MU0 * dx), rel=1e-15)
+    e0 = 1.7e3
+    prob["e_init"] = np.full(prob["num_gridpoints"], e0)
+    run = oracle.OracleRun(prob)
+    nsteps = 37
+    run.run(0, nsteps)
+    want = e0
+    for _ in range(nsteps):
+        want = fa * want
+    assert np.all(run.e == want)
+    assert np.all(run.h == 0.0)
+    t = nsteps * dt
+    cont = e0 * math.exp(-sigma * t / eps)
+    lead = nsteps * (2.0 / 3.0) * x ** 3 * (cont / e0)
+    assert abs((cont - want) / e0 - lead) <= 0.05 * lead
+    # the loss really acts: a factor-2 error in x would fail this
+    assert abs(want / e0 - math.exp(-sigma * t / eps)) < \
+        0.01 * abs(1 - math.exp(-sigma * t / eps))
+
+
+def test_curl_terms_of_linear_ramps():
+    """One step from E = 0 and H = i dH: interior E = fac_b dH / dx; from
+    E = i dE and H = 0: H = fac_c dE at the interior faces and zero at the
+    walls.  The signs are those of dE/dt = (dH/dx) / eps and
+    dH/dt = (dE/dx) / mu."""
+    eps_r = 3.0
+    prob = _line(200, {"rel_permittivity": eps_r})
+    fa, fb, fc, _ = prob["fdtd_consts"][:4]
+    dt, dx = prob["dt"], prob["dx"]
+    assert fa == 1.0
+    assert fb == pytest.approx(dt / (qm_setup.EPS0 * eps_r), rel=1e-15)
+    nx = prob["num_gridpoints"]
+    dh = 0.37
+    p1 = dict(prob)
+    p1["h_init"] = dh * np.arange(nx + 1, dtype=float)
+    p1["h_init"][0] = p1["h_init"][nx] = 0.0  # the walls
+    run = oracle.OracleRun(p1).run(0, 1)
+    inner = run.e[1:nx - 1]
+    assert np.allclose(inner, fb * dh / dx, rtol=1e-13, atol=0)
+    assert run.e[1] > 0
+    de = 2.5
+    p2 = dict(prob)
+    p2["e_init"] = de * np.arange(nx, dtype=float)
+    run = oracle.OracleRun(p2).run(0, 1)
+    assert np.allclose(run.h[1:nx], fc * de, rtol=1e-13, atol=0)
+    assert run.h[0] == 0.0 and run.h[nx] == 0.0
+
+
+# ------------------------------------------------ Yee stencil, boundary ----
+@pytest.mark.parametrize("m", [1, 17, 150])
+def test_yee_dispersion_of_standing_modes(m):
+    """Vacuum between the magnetic walls h[0] = h[N] = 0: the cosine modes
+    E_i = cos(k (i + 1/2) dx), k = m pi / (N dx), are the eigenmodes; every
+    cell's E then follows E^{n+1} + E^{n-1} = 2 cos(w dt) E^n with the Yee
+    dispersion relation sin(w dt / 2) / (c dt) = sin(k dx / 2) / dx, at the
+    walls as in the interior."""
+    nx = 400
+    prob = _line(nx, {}, endtime_steps=200)
+    dt, dx = prob["dt"], prob["dx"]
+    k = m * math.pi / (nx * dx)
+    prob["e_init"] = np.cos(k * (np.arange(nx) + 0.5) * dx)
+    run = oracle.OracleRun(prob)
+    cells = [0, 1, nx // 3, nx // 2 + 1, nx - 1]
+    hist = []
+    for n in range(60):
+        run.run(n, n + 1)
+        hist.append(run.e[cells].copy())
+    hist = np.array(hist)
+    s = C0 * dt / dx * math.sin(k * dx / 2)
+    cos_wdt = 1.0 - 2.0 * s * s
+    lhs = hist[2:] + hist[:-2]
+    rhs = 2.0 * cos_wdt * hist[1:-1]
+    assert np.abs(lhs - rhs).max() < 1e-12
+    # the continuous relation w = c k would be off by far more
+    cont = math.cos(C0 * k * dt)
+    if m >= 17:
+        assert np.abs(lhs - 2.0 * cont * hist[1:-1]).max() > 1e-9
+
+
+def test_magnetic_wall_reflects_e_with_unchanged_sign():
+    """A Gaussian E pulse (H = 0) splits into two halves; the right half
+    reflects at the right wall (H = 0 there: a magnetic wall, reflection
+    coefficient +1 for E) and comes back with a positive peak of half the
+    initial amplitude.  An electric wall (E = 0) would return it negated."""
+    nx = 1200
+    prob = _line(nx, {}, endtime_steps=2000)
+    dt, dx = prob["dt"], prob["dx"]
+    x = np.arange(nx)
+    x0, w = 900, 25.0
+    prob["e_init"] = np.exp(-((x - x0) / w) ** 2)
+    run = oracle.OracleRun(prob)
+    # the right half reaches the wall (cell nx - 1/2) after (nx - x0) dx / c
+    # and is back at x0 after twice that; the left half is then near 600
+    steps = int(round(2 * (nx - 0.5 - x0) * dx / (C0 * dt)))
+    run.run(0, steps)
+    right = run.e[750:]
+    assert right.max() == pytest.approx(0.5, abs=0.02)
+    assert right.min() > -0.02
+    assert 880 <= 750 + int(np.argmax(right)) <= 920
+    left = run.e[:750]
+    assert left.max() == pytest.approx(0.5, abs=0.02)
+
+
+# ------------------------------------------------------------- records -----
+def test_record_rows_of_the_sit_workload():
+    """The two-level SIT run (8192 cells, 1.5 ps, interval T/255) records
+    exactly 256 rows: floor(T / interval) + 1 rows and as many steps at
+    which floor(ratio n) changes (n = 0 included), the last at row 255."""
+    spec = setups.sit_2lvl()
+    prob = qm_setup.build_problem(spec)
+    nt = prob["num_timesteps"]
+    assert nt == 49114
+    for r in prob["records"]:
+        ratio, rows = r["interval_ratio"], r["rows"]
+        assert rows == 256
+        n = np.arange(nt, dtype=float)
+        idx = np.floor(ratio * n)
+        hits = 1 + int(np.count_nonzero(idx[1:] != idx[:-1]))
+        assert hits == rows
+        assert int(idx[-1]) == rows - 1
+
+
+@pytest.mark.parametrize("k", [3, 7, 10, 64])
+def test_record_rows_hold_the_state_after_their_step(k):
+    """A point record of E and a full-grid record of H with interval T/k:
+    row r holds the state after the first step n with floor(ratio n) = r
+    (n = 0 always records), checked against the state stepped one by one."""
+    prob = _line(120, {"rel_permittivity": 2.0}, endtime_steps=90)
+    nt = prob["num_timesteps"]
+    spec_T = prob["dt"] * (nt - 1)
+    nx = prob["num_gridpoints"]
+    prob["e_init"] = np.sin(np.arange(nx) * 0.3)
+    interval = spec_T / k
+    ratio = prob["dt"] / interval
+    rows = int(math.floor(spec_T / interval) + 1)
+    prob["records"] = [
+        {"type": 0, "row_idx": 0, "col_idx": 0, "is_complex": 0,
+         "rows": rows, "cols": 1, "position_idx": 37,
+         "interval_ratio": ratio, "offset_real": 0, "offset_imag": rows},
+        {"type": 2, "row_idx": 0, "col_idx": 0, "is_complex": 0,
+         "rows": rows, "cols": nx, "position_idx": 0,
+         "interval_ratio": ratio, "offset_real": rows,
+         "offset_imag": rows + rows * nx}]
+    prob["scratch_size"] = rows + rows * nx
+    rec_e, rec_h = oracle.OracleRun(prob).run().records()
+    run = oracle.OracleRun(prob)
+    seen = set()
+    for n in range(nt):
+        run.run(n, n + 1)
+        r = int(math.floor(ratio * n))
+        if (n == 0 or int(math.floor(ratio * (n - 1))) != r) and r < rows:
+            assert r not in seen
+            seen.add(r)
+            assert rec_e[r, 0] == run.e[37]
+            assert np.array_equal(rec_h[r], run.h[:nx])
+    assert seen == set(range(rows))
+
+
+# ------------------------------------------------------- Cayley step -------
+@pytest.mark.parametrize("n", [2, 3])
+def test_cayley_step_without_relaxation_is_the_cayley_rotation(n):
+    """With zero scattering and dephasing the oracle's step is
+    rho' = U rho U^H, U = (I - M/2)^-1 (I + M/2), M = -i dt/hbar (H0 - e mu)
+    (built here with numpy), unitary: eigenvalues and trace are kept."""
+    rng = np.random.default_rng(7 + n)
+    hd = list(rng.uniform(0, 3e-20, n))
+    ho = list(rng.standard_normal(n * (n - 1) // 2) * 1e-21)
+    md = [0.0] * n
+    mo = list(rng.standard_normal(n * (n - 1) // 2) * 1e-29)
+    q = qm_setup.qm_from_spec({
+        "num_levels": n, "carrier_density": 1e24, "h_diag": hd,
+        "h_off": ho, "mu_diag": md, "mu_off": mo,
+        "scattering": [[0.0] * n for _ in range(n)],
+        "pure_dephasing": [0.0] * (n * n)})
+    dt, e = 2e-17, 4e8
+    blob = qm_setup.cayley_constants(q, dt, n)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    rho = a @ a.conj().T
+    rho /= np.trace(rho).real
+    d = np.column_stack([rho.real.ravel(), rho.imag.ravel()]).ravel()
+    d2, _ = oracle.cayley_update(n, blob, d, e)
+    got = (d2[0::2] + 1j * d2[1::2]).reshape(n, n)
+    H = qm_setup.to_matrix(q["h_diag"], q["h_off"], n)
+    mu = qm_setup.to_matrix(q["mu_diag"], q["mu_off"], n)
+    M = -1j * dt / qm_setup.HBAR * (H - e * mu)
+    U = np.linalg.solve(np.eye(n) - 0.5 * M, np.eye(n) + 0.5 * M)
+    want = U @ rho @ U.conj().T
+    assert np.abs(got - want).max() < 1e-14
+    assert np.abs(U @ U.conj().T - np.eye(n)).max() < 1e-14
+    assert np.allclose(np.linalg.eigvalsh(got), np.linalg.eigvalsh(rho),
+                       atol=1e-14)
+    # the transposed rotation would differ
+    wrong = U.conj().T @ rho @ U
+    assert np.abs(got - wrong).max() > 1e-10
+
+
+# ------------------------------------------------- marskar2011 curves ------
+def marskar_point_problem(num_levels=6, cells=64):
+    """marskar2011 (mbsolve-tool.cpp:479-662) with the tool's grid spacing
+    (1 mm / 8191) on `cells` cells: the records sit at x = 0, the hard
+    source's cell, whose density sees only the source, so its records
+    equal those of the 8192-cell run (same dt, same n_t)."""
+    spec = setups.marskar2011(num_levels)
+    dx = 1e-3 / 8191
+    spec["num_gridpoints"] = cells
+    spec["regions"] = [("Active region", "AR_Marsk", 0.0, (cells - 1) * dx)]
+    return spec
+
+
+def check_marskar_curves(recs, dt):
+    """d55 (record 4) and d66 (record 5) against the digitized curves:
+    features of a plot digitization (start, peak, end, half-rise time)."""
+    out = {}
+    for k, fn in ((4, "marskar2011-rho55.csv"), (5, "marskar2011-rho66.csv")):
+        ref = np.loadtxt(os.path.join(HERE, "golden", "reference-data", fn),
+                         delimiter=",")
+        sim = recs[k][:, 0]
+        t = np.arange(len(sim)) * dt
+        s = np.interp(ref[:, 0] * 1e-12, t, sim)
+        d = s - ref[:, 1]
+        assert abs(s[0] - ref[0, 1]) < 2e-3, fn
+        assert abs(sim.max() - ref[:, 1].max()) < 0.01 * ref[:, 1].max(), fn
+        assert abs(s[-1] - ref[-1, 1]) < 0.01 * ref[:, 1].max(), fn
+        half = 0.5 * (ref[0, 1] + ref[:, 1].max())
+        t_ref = ref[np.argmax(ref[:, 1] > half), 0] * 1e-12
+        t_sim = t[np.argmax(sim > half)]
+        assert abs(t_sim - t_ref) < 10e-15, (fn, t_sim, t_ref)
+        assert np.sqrt(np.mean(d * d)) < 0.05 * ref[:, 1].max(), fn
+        out[fn] = float(np.abs(d).max())
+    return out
+
+
+def test_marskar2011_reference_curves():
+    spec = marskar_point_problem()
+    prob = qm_setup.build_problem(spec)
+    assert prob["num_timesteps"] == 9824
+    recs = oracle.OracleRun(prob).run().records()
+    # the populations of all six levels sum to one at every step
+    tot = sum(recs[k][:, 0] for k in range(6))
+    assert np.abs(tot - 1.0).max() < 1e-12
+    check_marskar_curves(recs, prob["dt"])
+
+
+def test_marskar_point_records_do_not_depend_on_the_grid_length():
+    """The reduction above: the x = 0 records of a 16-cell and a 64-cell
+    device with the same grid spacing are identical (four levels, to keep
+    it short)."""
+    recs = []
+    for cells in (16, 64):
+        prob = qm_setup.build_problem(marskar_point_problem(4, cells))
+        recs.append(oracle.OracleRun(prob).run(0, 1500).records())
+    for a, b in zip(*recs):
+        assert np.array_equal(a, b)
