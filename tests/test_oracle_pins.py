@@ -13,7 +13,9 @@
 * the regularized Cayley step (algo_lindblad_reg_cayley.hpp:219-257)
   without relaxation against U rho U^H with the Cayley factor built here;
 * the marskar2011 six-level set-up (mbsolve-tool.cpp:479-662) against the
-  digitized rho55 / rho66 curves of tools/reference-data.
+  digitized rho55 / rho66 curves of tools/reference-data;
+* the source tables (source.hpp:94-106,190-194,256-260) against the closed
+  integrals of their envelopes (the 2 pi area of the SIT pulse).
 """
 import math
 import os
@@ -339,3 +341,41 @@ def test_marskar_point_records_do_not_depend_on_the_grid_length():
         recs.append(oracle.OracleRun(prob).run(0, 1500).records())
     for a, b in zip(*recs):
         assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------ sources ------
+def test_sech_source_table_carries_a_two_pi_pulse():
+    """The two-level SIT source table (sech_pulse, source.hpp:190-194):
+    sum table^2 dt = A^2 / beta (sech^2 integrates to 2 / beta, the carrier
+    sin^2 averages 1/2), and the envelope area mu A pi / (hbar beta) of that
+    A is the 2 pi of self-induced transparency."""
+    spec = setups.sit_2lvl()
+    prob = qm_setup.build_problem(spec)
+    src = spec["sources"][0]
+    table, dt = prob["source_data"], prob["dt"]
+    beta = src["beta"]
+    amp = math.sqrt(beta * float(np.sum(table * table)) * dt)
+    assert amp == pytest.approx(src["ampl"], rel=1e-4)
+    mu = spec["materials"]["AR_sit2"]["qm"]["dipole_moment"] * qm_setup.E0
+    area = mu * amp * math.pi / (qm_setup.HBAR * beta)
+    assert area == pytest.approx(2 * math.pi, rel=1e-4)
+    # the envelope peaks at t0 = phase / beta
+    t = np.arange(len(table)) * dt
+    assert abs(t[np.argmax(np.abs(table))] - src["phase"] / beta) < \
+        0.5 / src["freq"]
+
+
+def test_gaussian_source_table_energy():
+    """gaussian_pulse (source.hpp:256-260) of the marskar2011 set-up:
+    sum table^2 dt = A^2 tau sqrt(pi / 2) / 2 and the peak at t = phase."""
+    spec = marskar_point_problem()
+    prob = qm_setup.build_problem(spec)
+    src = spec["sources"][0]
+    table, dt = prob["source_data"], prob["dt"]
+    tau = src["tau"]
+    amp = math.sqrt(2.0 * float(np.sum(table * table)) * dt /
+                    (tau * math.sqrt(math.pi / 2)))
+    assert amp == pytest.approx(src["ampl"], rel=1e-3)
+    t = np.arange(len(table)) * dt
+    assert abs(t[np.argmax(np.abs(table))] - src["phase"]) < \
+        0.5 / src["freq"]
