@@ -667,13 +667,12 @@ uint32_t kernel_form(uint32_t algo, uint32_t levels, uint32_t form)
     return form;
 }
 
-/* The VALU z-form with FP32 storage keeps y - y* in memory (y* appended to
- * the device blobs), so the storage rounding scales with the distance from
- * the relaxation equilibrium; the MFMA forms keep y. */
+/* The z-forms with FP32 storage (VALU and MFMA) keep y - y* in memory (y*
+ * appended to the device blobs), so the storage rounding scales with the
+ * distance from the relaxation equilibrium. */
 uint32_t yshift(const mbh_ctx* ctx)
 {
-    return (ctx->zform && ctx->fp32 && ctx->form == MBH_FORM_MIXED) ? 1u
-                                                                    : 0u;
+    return (ctx->zform && ctx->fp32) ? 1u : 0u;
 }
 
 /* Cells owned by tiles: the trailing pseudo cell is never written. */
@@ -802,6 +801,8 @@ KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
                 uint32_t nsteps);
 hipError_t launch_block(const mbh_ctx* ctx, KArgs k);
 
+constexpr uint32_t FP32_MIN_BLOCK = 8;
+
 /*
  * Autotuner: times every admissible (cells per lane, temporal block) pair
  * on the context's own state and picks the fastest per cell update.  The
@@ -827,6 +828,13 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
         return fail(ctx, MBH_ERR_UNSUPPORTED,
                     "no kernel with %u cells per wavefront", req_tile);
     }
+    /* FP32 storage rounds the state at every launch boundary, and the
+     * error of a run grows with the number of those roundings: automatic
+     * blocks of these forms are at least FP32_MIN_BLOCK steps deep (where
+     * the run is that long) */
+    const uint32_t min_tb = (ctx->fp32 && ctx->n_t >= 4u * FP32_MIN_BLOCK)
+        ? FP32_MIN_BLOCK
+        : 1u;
     std::vector<std::pair<uint32_t, uint32_t>> cand;
     for (uint32_t C : cs) {
         for (uint32_t tb : { 1u, 2u, 4u, 6u, 8u, 12u, 16u, 20u, 24u, 28u,
@@ -837,7 +845,8 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
             if (tb > MAX_BLOCK || 64u * C < 2u * tb + 8u) {
                 continue;
             }
-            if (!req_tb && tb > std::max<uint64_t>(1, ctx->n_t)) {
+            if (!req_tb && (tb > std::max<uint64_t>(1, ctx->n_t) ||
+                            tb < min_tb)) {
                 continue;
             }
             cand.emplace_back(C, tb);
@@ -1315,8 +1324,9 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
             if (c->zform) {
                 double* dtmp = nullptr;
                 RC(upload(c, &dtmp, d.data(), d.size()));
-                ZArgs za{ dtmp, c->state[0][3], G, c->dstride, c->mat, c->qm,
-                          c->qstride, yshift(c) };
+                ZArgs za{ dtmp,    c->state[0][3], G, c->dstride, c->mat,
+                          c->qm,   c->qstride,     yshift(c), 0, G,
+                          nullptr };
                 HIP_TRY(c, c->to_y(za, c->stream));
                 HIP_TRY(c, hipStreamSynchronize(c->stream));
             } else if (c->fp32) {
@@ -1631,8 +1641,10 @@ int mbh_fetch_state(mbh_ctx* ctx, uint32_t run, double* e, double* h,
             if (rc) {
                 return rc;
             }
-            ZArgs za{ dtmp, S[3], ctx->G, ctx->dstride, ctx->mat, ctx->qm,
-                      ctx->qstride, yshift(ctx) };
+            ZArgs za{ dtmp,         S[3],        ctx->G,
+                      ctx->dstride, ctx->mat,    ctx->qm,
+                      ctx->qstride, yshift(ctx), 0,
+                      ctx->G,       nullptr };
             HIP_TRY(ctx, ctx->from_y(za, ctx->stream));
             HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
             for (uint32_t j = 0; j < ctx->dref; j++) {

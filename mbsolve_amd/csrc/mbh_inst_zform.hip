@@ -3,7 +3,7 @@
  * (MBH_FORM_MIXED) or FP64 (MBH_FORM_ZFORM64) state storage, plus the
  * conversions between the reference coherence vector d and the z-form
  * state y:  y = Q^T A1 (d + d_in) - Q^T d_in,   d = P y + o
- * (FP32 storage of the VALU z-form holds y - y*, y* = -Q^T d_in).
+ * (FP32 storage of the z-forms holds y - y*, y* = -Q^T d_in).
  */
 #include "mbh_tile.hpp"
 
@@ -13,8 +13,9 @@ template<int N, typename ST>
 __global__ __launch_bounds__(64) void zform_to_y(ZArgs a)
 {
     constexpr int K = N * N - 1;
-    const uint64_t g = (uint64_t)blockIdx.x * 64 + threadIdx.x;
-    const bool in = g < a.G;
+    const uint64_t i = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+    const uint64_t g = a.g0 + i;
+    const bool in = i < a.count;
     const int32_t m = in ? a.mat[g] : -1;
     double d[K], y[K];
 #pragma unroll
@@ -64,8 +65,9 @@ template<int N, typename ST>
 __global__ __launch_bounds__(64) void zform_from_y(ZArgs a)
 {
     constexpr int K = N * N - 1;
-    const uint64_t g = (uint64_t)blockIdx.x * 64 + threadIdx.x;
-    const bool in = g < a.G;
+    const uint64_t i = (uint64_t)blockIdx.x * 64 + threadIdx.x;
+    const uint64_t g = a.g0 + i;
+    const bool in = i < a.count;
     const int32_t m = in ? a.mat[g] : -1;
     typename algo_zform<N>::state s;
     double d[K];
@@ -90,7 +92,12 @@ __global__ __launch_bounds__(64) void zform_from_y(ZArgs a)
             }
         }
     });
-    if (in) {
+    if (in && a.out_cm) {
+#pragma unroll
+        for (int j = 0; j < K; j++) {
+            a.out_cm[i * K + j] = d[j];
+        }
+    } else if (in) {
 #pragma unroll
         for (int j = 0; j < K; j++) {
             const_cast<double*>(a.dref)[(uint64_t)j * a.dstride + g] = d[j];
@@ -101,16 +108,17 @@ __global__ __launch_bounds__(64) void zform_from_y(ZArgs a)
 template<int N, typename ST>
 hipError_t launch_to_y(const ZArgs& a, hipStream_t stream)
 {
-    hipLaunchKernelGGL((zform_to_y<N, ST>), dim3((unsigned)((a.G + 63) / 64)),
-                       dim3(64), 0, stream, a);
+    hipLaunchKernelGGL((zform_to_y<N, ST>),
+                       dim3((unsigned)((a.count + 63) / 64)), dim3(64), 0,
+                       stream, a);
     return hipGetLastError();
 }
 template<int N, typename ST>
 hipError_t launch_from_y(const ZArgs& a, hipStream_t stream)
 {
     hipLaunchKernelGGL((zform_from_y<N, ST>),
-                       dim3((unsigned)((a.G + 63) / 64)), dim3(64), 0, stream,
-                       a);
+                       dim3((unsigned)((a.count + 63) / 64)), dim3(64), 0,
+                       stream, a);
     return hipGetLastError();
 }
 

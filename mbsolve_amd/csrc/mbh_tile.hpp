@@ -83,7 +83,7 @@ struct KArgs {
     uint32_t nsteps;
     uint32_t halo;
     uint32_t owned;
-    uint32_t yshift; /* FP32 z-form storage holds y - y* (VALU z-form) */
+    uint32_t yshift; /* FP32 z-form storage holds y - y* (VALU and MFMA) */
     const void* frag;     /* MFMA form: [nmat][frag_stride bytes] */
     uint64_t frag_stride;
     const uint32_t* tile_list; /* tiles of this launch (NULL: blockIdx) */
@@ -578,6 +578,10 @@ struct ZArgs {
     const double* zq;
     uint64_t zstride;
     uint32_t yshift; /* FP32 storage holds y - y* */
+    uint64_t g0;     /* first super-grid cell converted */
+    uint64_t count;  /* cells converted */
+    double* out_cm;  /* y -> d only: if set, d of cell g0 + i goes to
+                        out_cm[i * K + j] (cell-major) instead of dref */
 };
 typedef hipError_t (*zconv_fn)(const ZArgs& a, hipStream_t stream);
 

@@ -359,6 +359,25 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
     const int32_t m0 = m;
     const char* fbase = static_cast<const char*>(k.frag);
     const bool q0_active = (tm & TILE_MAT_QM) != 0;
+    /* FP32 storage holds y - y* (y* = -Q^T d_in, the relaxation
+     * equilibrium, at OFF_YEQ of the material's z-form blob): the stored
+     * value and its rounding scale with the distance from equilibrium */
+    const double* ystar = (const double*)k.qm + (uint64_t)m0 * k.qstride +
+        algo_zform<N>::OFF_YEQ;
+    const bool shifted =
+        std::is_same<ST, float>::value && k.yshift && q0_active;
+    if (shifted) {
+#pragma unroll
+        for (int n = 0; n < Gm::NB; n++) {
+#pragma unroll
+            for (int u = 0; u < Gm::ROWS; u++) {
+                const int row = Gm::slot_row(q, u);
+                if (row < Gm::K) {
+                    Y[n][u] = fma(1.0, ystar[row], Y[n][u]);
+                }
+            }
+        }
+    }
     __shared__ zm_h8 s_frag[Gm::NF][64];
     zm_consts<N> KC0;
     if (q0_active) {
@@ -545,7 +564,10 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
             for (int u = 0; u < Gm::ROWS; u++) {
                 const int row = Gm::slot_row(q, u);
                 if (row < Gm::K) {
-                    st<ST>(k.d_out, (uint64_t)row * k.dstride + gn, Y[n][u]);
+                    const double v = shifted
+                        ? fma(-1.0, ystar[row], Y[n][u])
+                        : Y[n][u];
+                    st<ST>(k.d_out, (uint64_t)row * k.dstride + gn, v);
                 }
             }
         }
