@@ -317,6 +317,26 @@ int mbh_fetch_records(mbh_ctx* ctx, uint32_t run, double* dst,
 int mbh_fetch_state(mbh_ctx* ctx, uint32_t run, double* e, double* h,
                     double* p, double* d);
 
+/* Size of the per-GPU staging array of mbh_fetch_state_noalloc. */
+#define MBH_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* As mbh_fetch_state, without allocating: no counterpart in the reference.
+ * For callers that cannot allocate memory while they read the state (a
+ * monitoring loop between mbh_run calls, harnesses that forbid hipMalloc).
+ * mbh_fetch_state allocates a device buffer for the z-form read-out and
+ * host buffers for FP32 widening and the density transposition; this entry
+ * point instead stages every conversion (FP32 -> FP64, z-form or Bloch
+ * state -> the reference density layout, component-major -> cell-major)
+ * in one fixed device array of MBH_NOALLOC_SCRATCH_BYTES per GPU, held
+ * from the first call on that GPU until the process ends, and copies from
+ * there into the caller's host arrays.  The results are bit-identical to
+ * mbh_fetch_state.  A run whose largest staged array (N_x + 1 doubles, or
+ * N_x times the density stride when d is requested) exceeds the array is
+ * rejected with MBH_ERR_INVALID.  Calls on one GPU share the array: calls
+ * that overlap (several threads or contexts on one GPU at once) race. */
+int mbh_fetch_state_noalloc(mbh_ctx* ctx, uint32_t run, double* e, double* h,
+                            double* p, double* d);
+
 /* Describes the chosen launch configuration. */
 int mbh_get_info(const mbh_ctx* ctx, mbh_info* info);
 

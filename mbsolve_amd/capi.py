@@ -28,7 +28,8 @@ FORMS = {"fp64": MBH_FORM_FP64, "mixed": MBH_FORM_MIXED,
 EXPORTED = ["mbh_version", "mbh_device_count", "mbh_default_options",
             "mbh_create", "mbh_run", "mbh_reset", "mbh_synchronize",
             "mbh_elapsed_ms",
-            "mbh_fetch_records", "mbh_fetch_state", "mbh_get_info",
+            "mbh_fetch_records", "mbh_fetch_state",
+            "mbh_fetch_state_noalloc", "mbh_get_info",
             "mbh_destroy", "mbh_last_error"]
 
 
@@ -80,6 +81,7 @@ class mbh_info(C.Structure):
                 ("mfma_tiles", C.c_uint32), ("flags", C.c_uint32)]
 
 MBH_INFO_HZ_ZERO = 1
+MBH_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
 _lib = None
@@ -116,6 +118,9 @@ def load():
     lib.mbh_fetch_records.argtypes = [vp, C.c_uint32, _dp, C.c_uint64]
     lib.mbh_fetch_state.restype = C.c_int
     lib.mbh_fetch_state.argtypes = [vp, C.c_uint32, _dp, _dp, _dp, _dp]
+    lib.mbh_fetch_state_noalloc.restype = C.c_int
+    lib.mbh_fetch_state_noalloc.argtypes = [vp, C.c_uint32, _dp, _dp, _dp,
+                                            _dp]
     lib.mbh_get_info.restype = C.c_int
     lib.mbh_get_info.argtypes = [vp, C.POINTER(mbh_info)]
     lib.mbh_destroy.restype = None
@@ -251,16 +256,19 @@ class Context:
             self.ctx, run, out.ctypes.data_as(_dp), n))
         return out[:n]
 
-    def state(self, run=0):
+    def state(self, run=0, noalloc=False):
+        """Current state of `run` (mbh_fetch_state, or with noalloc=True
+        mbh_fetch_state_noalloc)."""
         d = self.problems[run].dict
         nx = int(d["num_gridpoints"])
         ds = len(d["d_init"]) // nx if nx else 0
         e, h, p = np.zeros(nx), np.zeros(nx + 1), np.zeros(nx)
         dd = np.zeros(max(nx * ds, 1))
-        self._check(self.lib.mbh_fetch_state(
-            self.ctx, run, e.ctypes.data_as(_dp), h.ctypes.data_as(_dp),
-            p.ctypes.data_as(_dp),
-            dd.ctypes.data_as(_dp) if ds else None))
+        fn = self.lib.mbh_fetch_state_noalloc if noalloc else \
+            self.lib.mbh_fetch_state
+        self._check(fn(self.ctx, run, e.ctypes.data_as(_dp),
+                       h.ctypes.data_as(_dp), p.ctypes.data_as(_dp),
+                       dd.ctypes.data_as(_dp) if ds else None))
         return {"e": e, "h": h, "p": p, "d": dd[:nx * ds]}
 
     def info(self):

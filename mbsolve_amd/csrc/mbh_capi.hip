@@ -16,9 +16,20 @@
 #include <string>
 #include <vector>
 
+#include <mutex>
+
 #include "mbh_zmfma.hpp"
 
 using namespace mbh;
+
+/*
+ * Staging array of mbh_fetch_state_noalloc: one copy per GPU (placed when
+ * this code object is loaded on that GPU, held until the process ends).
+ * Host code never uses the name itself for device memory: the per-device
+ * address comes from hipGetSymbolAddress (noalloc_scratch below).
+ */
+static __device__ __align__(256) unsigned char
+    scratch[MBH_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -35,6 +46,76 @@ __global__ void zero_last_h(ST* h0, ST* h1, uint64_t g)
 {
     h0[g] = ST(0);
     h1[g] = ST(0);
+}
+
+/* state read-out without allocation (mbh_fetch_state_noalloc) */
+template<typename ST>
+__global__ void widen_range(const ST* __restrict__ src,
+                            double* __restrict__ dst, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
+        dst[i] = (double)src[i];
+    }
+}
+
+/* component-major density [K][dstride] of cells [base, base + cells) ->
+ * cell-major [cells][K] */
+template<typename ST>
+__global__ void gather_cell_major(const ST* __restrict__ d, uint64_t dstride,
+                                  uint64_t base, uint64_t cells, uint32_t K,
+                                  double* __restrict__ out)
+{
+    const uint64_t n = cells * K;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += stride) {
+        const uint64_t j = t / cells, i = t - j * cells;
+        out[i * K + j] = (double)d[j * dstride + base + i];
+    }
+}
+
+/* carried Bloch state -> reference 2x2 complex layout (the device twin of
+ * bloch2_from_sigma plus the Hermitian fill of mbh_fetch_state) */
+template<typename ST>
+__global__ void bloch_to_ref(const ST* __restrict__ sig, uint64_t dstride,
+                             uint64_t base, uint64_t cells,
+                             const int32_t* __restrict__ mat,
+                             const double* __restrict__ qm, uint64_t qstride,
+                             double* __restrict__ out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+         i < cells; i += stride) {
+        const uint64_t g = base + i;
+        const double* o = qm + (uint64_t)mat[g] * qstride;
+        double sv[4], rho[4];
+        for (int j = 0; j < 4; j++) {
+            sv[j] = (double)sig[(uint64_t)j * dstride + g];
+        }
+        if (o[0] == 0.0) {
+            for (int j = 0; j < 4; j++) {
+                rho[j] = sv[j];
+            }
+        } else {
+            const double* R = o + 18;
+            const double gg = o[26];
+            rho[0] = R[0] * sv[0] + R[1] * sv[3];
+            rho[1] = R[2] * sv[0] + R[3] * sv[3];
+            rho[2] = gg * sv[1];
+            rho[3] = -gg * sv[2];
+        }
+        double* di = out + i * 8;
+        di[0] = rho[0];
+        di[1] = 0.0;
+        di[2] = rho[2];
+        di[3] = rho[3];
+        di[4] = rho[2];
+        di[5] = -rho[3];
+        di[6] = rho[1];
+        di[7] = 0.0;
+    }
 }
 
 __global__ void fill_zero_f64(double* p, uint64_t n)
@@ -487,6 +568,33 @@ int dalloc(mbh_ctx* ctx, void** p, size_t bytes)
                     hipGetErrorString(e));
     }
     ctx->allocs.push_back(*p);
+    return MBH_OK;
+}
+
+/* Device address of `scratch` on the context's GPU (current), looked up
+ * once per GPU and kept for the process. */
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = { nullptr };
+
+int noalloc_scratch(mbh_ctx* ctx, double** out)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) {
+        return fail(ctx, MBH_ERR_UNSUPPORTED,
+                    "no staging array for device %d", ctx->device);
+    }
+    if (!g_noalloc_ptr[ctx->device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            return fail(ctx, MBH_ERR_HIP,
+                        "hipGetSymbolAddress(scratch) on device %d: %s",
+                        ctx->device, hipGetErrorString(e));
+        }
+        g_noalloc_ptr[ctx->device] = p;
+    }
+    *out = static_cast<double*>(g_noalloc_ptr[ctx->device]);
     return MBH_OK;
 }
 
@@ -1669,6 +1777,101 @@ int mbh_fetch_state(mbh_ctx* ctx, uint32_t run, double* e, double* h,
                     d[i * ctx->dref + j] = comp[i];
                 }
             }
+        }
+    }
+    return MBH_OK;
+}
+
+int mbh_fetch_state_noalloc(mbh_ctx* ctx, uint32_t run, double* e, double* h,
+                            double* p, double* d)
+{
+    if (!ctx) {
+        return fail(nullptr, MBH_ERR_INVALID, "ctx is NULL");
+    }
+    if (run >= ctx->runs.size()) {
+        return fail(ctx, MBH_ERR_INVALID, "run %u out of range", run);
+    }
+    const run_layout& L = ctx->runs[run];
+    const uint64_t cap = MBH_NOALLOC_SCRATCH_BYTES / sizeof(double);
+    const uint64_t need =
+        std::max<uint64_t>(L.cells + 1,
+                           (d && ctx->dref) ? L.cells * ctx->dref : 0);
+    if (need > cap) {
+        return fail(ctx, MBH_ERR_INVALID,
+                    "run %u needs %llu bytes of staging, more than the "
+                    "%llu of mbh_fetch_state_noalloc (use mbh_fetch_state)",
+                    run, (unsigned long long)(need * sizeof(double)),
+                    (unsigned long long)MBH_NOALLOC_SCRATCH_BYTES);
+    }
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    double* stage = nullptr;
+    int rc = noalloc_scratch(ctx, &stage);
+    if (rc) {
+        return rc;
+    }
+    /* everything on the context's stream, after the queued steps */
+    const dim3 grid(256), block(256);
+    void* const* S = ctx->state[ctx->cur];
+    auto copy_out = [&](double* dst, const double* src,
+                        uint64_t count) -> int {
+        HIP_TRY(ctx, hipMemcpyAsync(dst, src, count * sizeof(double),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+        return MBH_OK;
+    };
+    auto field = [&](const void* src, uint64_t count, double* dst) -> int {
+        const uint64_t first = L.base;
+        if (ctx->fp32) {
+            hipLaunchKernelGGL(widen_range<float>, grid, block, 0,
+                               ctx->stream, (const float*)src + first, stage,
+                               count);
+            HIP_TRY(ctx, hipGetLastError());
+            return copy_out(dst, stage, count);
+        }
+        /* FP64 state: straight from the state buffer */
+        return copy_out(dst, (const double*)src + first, count);
+    };
+    if (e && (rc = field(S[0], L.cells, e))) {
+        return rc;
+    }
+    if (h && (rc = field(S[1], L.cells + 1, h))) {
+        return rc;
+    }
+    if (p && (rc = field(S[2], L.cells, p))) {
+        return rc;
+    }
+    if (d && ctx->dref) {
+        if (ctx->bloch) {
+            if (ctx->fp32) {
+                hipLaunchKernelGGL(bloch_to_ref<float>, grid, block, 0,
+                                   ctx->stream, (const float*)S[3],
+                                   ctx->dstride, L.base, L.cells, ctx->mat,
+                                   ctx->qm, ctx->qstride, stage);
+            } else {
+                hipLaunchKernelGGL(bloch_to_ref<double>, grid, block, 0,
+                                   ctx->stream, (const double*)S[3],
+                                   ctx->dstride, L.base, L.cells, ctx->mat,
+                                   ctx->qm, ctx->qstride, stage);
+            }
+        } else if (ctx->zform) {
+            ZArgs za{ nullptr,      S[3],        ctx->G,
+                      ctx->dstride, ctx->mat,    ctx->qm,
+                      ctx->qstride, yshift(ctx), L.base,
+                      L.cells,      stage };
+            HIP_TRY(ctx, ctx->from_y(za, ctx->stream));
+        } else if (ctx->fp32) {
+            hipLaunchKernelGGL(gather_cell_major<float>, grid, block, 0,
+                               ctx->stream, (const float*)S[3], ctx->dstride,
+                               L.base, L.cells, ctx->dref, stage);
+        } else {
+            hipLaunchKernelGGL(gather_cell_major<double>, grid, block, 0,
+                               ctx->stream, (const double*)S[3],
+                               ctx->dstride, L.base, L.cells, ctx->dref,
+                               stage);
+        }
+        HIP_TRY(ctx, hipGetLastError());
+        if ((rc = copy_out(d, stage, L.cells * ctx->dref))) {
+            return rc;
         }
     }
     return MBH_OK;
