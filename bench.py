@@ -136,13 +136,37 @@ class Bench:
         self.info = self.sol.info()
         self.cells = int(self.info["total_cells"])
 
-    def one_pass(self, steps):
+    def one_pass(self, steps, chunk_s=20.0):
         """Rewinds to the initial state and runs steps [0, steps); returns
         the device time (HIP events on the solver's stream around the
-        launches) in ms after waiting for it."""
+        launches) in ms after waiting for it.  A pass expected to take
+        longer than about chunk_s runs in chunks of whole temporal blocks
+        (the same launches), with a progress line after each."""
         self.sol.reset()
-        self.sol.run_steps(0, steps)
-        return self.sol.elapsed_ms()
+        tb = max(1, int(self.info["temporal_block"]))
+        per_step = getattr(self, "_ms_per_step", 0.0)
+        chunk = steps
+        if per_step > 0.0 and per_step * steps > chunk_s * 1e3:
+            chunk = max(tb, int(chunk_s * 1e3 / per_step) // tb * tb)
+        dev, n = 0.0, 0
+        while n < steps:
+            m = min(steps, n + chunk)
+            self.sol.run_steps(n, m)
+            dev += self.sol.elapsed_ms()
+            n = m
+            if n < steps:
+                log("    %d of %d steps, %.1f s device" % (n, steps,
+                                                           dev * 1e-3))
+        self._ms_per_step = dev / max(steps, 1)
+        return dev
+
+    def estimate(self, steps=None):
+        """Device time per step from a short probe pass (for chunking)."""
+        tb = max(1, int(self.info["temporal_block"]))
+        k = min(self.n_t, max(8 * tb, 256) if steps is None else steps)
+        ms = self.one_pass(k)
+        self._ms_per_step = ms / k
+        return self._ms_per_step
 
     def records(self):
         """[[record arrays of run r] for r] after the last pass."""
@@ -429,6 +453,8 @@ def main():
         log("reference form '%s' on %s" % (ref_form or "fp64", workload))
         b = Bench(specs, ref_form, opts)
         n_pass = min(pass_steps_req or b.n_t, b.n_t)
+        if n_pass > 100000:
+            b.estimate()
         ms = b.one_pass(n_pass)
         ref_recs, ref_names = b.records(), b.record_names()
         probs = [b.problem(r) for r in range(len(specs))]
@@ -453,6 +479,8 @@ def main():
         log("form %s on %s" % (form, workload))
         b = Bench(specs, form, opts)
         n_pass = min(pass_steps_req or b.n_t, b.n_t)
+        if n_pass > 100000:
+            b.estimate()
         ms = b.one_pass(n_pass)
         r = {"value": b.cells * n_pass / (ms * 1e-3), "dev_ms": ms,
              "info": b.info, "steps": n_pass, "solver": b.name}

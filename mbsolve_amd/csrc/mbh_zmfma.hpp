@@ -40,7 +40,9 @@ namespace mbh {
 #endif
 
 typedef _Float16 zm_h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 zm_h2 __attribute__((ext_vector_type(2)));
 typedef float zm_f4 __attribute__((ext_vector_type(4)));
+typedef float zm_f2 __attribute__((ext_vector_type(2)));
 
 /* geometry and per-material constant layout (bytes) of the MFMA form */
 template<int N>
@@ -222,18 +224,32 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
             pa = fma(K_.ar[i], Y[n][i], pa);
         }
         pp[n] = pa;
+        /* r = hi + lo: hi = f16(r) and lo = f16(r - hi), both rounded to
+         * nearest, converted two at a time (v_cvt_pk_f16_f32); r - hi is
+         * exact in FP32 and is formed from the packed half directly
+         * (mixed-precision FMA) */
         zm_h8 bh[Gm::KC], bl[Gm::KC];
 #pragma unroll
         for (int kc = 0; kc < Gm::KC; kc++) {
 #pragma unroll
-            for (int sl = 0; sl < 8; sl++) {
-                const int u = 8 * kc + sl;
-                const float v = (u < Gm::ROWS)
-                    ? (float)Y[n][u]
-                    : ((Gm::CFOLD && u == Gm::ROWS) ? one_q0 : 0.0f);
-                const _Float16 hi = (_Float16)v;
-                bh[kc][sl] = hi;
-                bl[kc][sl] = (_Float16)(v - (float)hi);
+            for (int sl = 0; sl < 8; sl += 2) {
+                zm_f2 v;
+#pragma unroll
+                for (int w = 0; w < 2; w++) {
+                    const int u = 8 * kc + sl + w;
+                    v[w] = (u < Gm::ROWS)
+                        ? (float)Y[n][u]
+                        : ((Gm::CFOLD && u == Gm::ROWS) ? one_q0 : 0.0f);
+                }
+                const zm_h2 hi = __builtin_convertvector(v, zm_h2);
+                zm_f2 rest;
+                rest[0] = __builtin_fmaf((float)hi[0], -1.0f, v[0]);
+                rest[1] = __builtin_fmaf((float)hi[1], -1.0f, v[1]);
+                const zm_h2 lo = __builtin_convertvector(rest, zm_h2);
+                bh[kc][sl] = hi[0];
+                bh[kc][sl + 1] = hi[1];
+                bl[kc][sl] = lo[0];
+                bl[kc][sl + 1] = lo[1];
             }
         }
         /* (W - I) r of this block = A_hi r_hi + A_hi r_lo [+ A_mid r_hi]
