@@ -244,8 +244,16 @@ typedef struct mbh_options {
                                 graph otherwise (mbh_info.graph_launches
                                 reports the choice) */
     uint32_t step_order;     /* MBH_ORDER_* */
-    uint32_t reserved[2];
+    uint32_t flags;          /* MBH_OPT_* */
+    uint32_t reserved[1];
 } mbh_options;
+
+/* mbh_options.flags: run the general kernels of the form even where the
+ * problem admits a specialised one (pure Maxwell with every held H cell
+ * starting at zero; two-level Bloch form with a diagonal Hamiltonian and a
+ * real dipole).  The specialised kernels drop terms that are exact zeros,
+ * so the results are the same; this flag is for verifying that. */
+#define MBH_OPT_GENERIC 1u
 
 typedef struct mbh_info {
     uint32_t form;
@@ -266,6 +274,9 @@ typedef struct mbh_info {
 /* mbh_info.flags: the pure-Maxwell FP32 kernel without the H factor of
  * the cells held at zero is in use (every such cell starts with H = 0) */
 #define MBH_INFO_HZ_ZERO 1u
+/* mbh_info.flags: the two-level Bloch kernel specialised for beta_y = 0 and
+ * a field-free alpha0 (diagonal Hamiltonian, real dipole) is in use */
+#define MBH_INFO_BLOCH_REAL 2u
 
 typedef struct mbh_ctx mbh_ctx;
 

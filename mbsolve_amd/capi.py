@@ -69,7 +69,9 @@ class mbh_options(C.Structure):
     _fields_ = [("form", C.c_uint32), ("temporal_block", C.c_uint32),
                 ("tile_cells", C.c_uint32), ("device", C.c_int32),
                 ("graph_launches", C.c_uint32), ("step_order", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("flags", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+
+MBH_OPT_GENERIC = 1
 
 
 class mbh_info(C.Structure):
@@ -81,6 +83,7 @@ class mbh_info(C.Structure):
                 ("mfma_tiles", C.c_uint32), ("flags", C.c_uint32)]
 
 MBH_INFO_HZ_ZERO = 1
+MBH_INFO_BLOCH_REAL = 2
 MBH_NOALLOC_SCRATCH_BYTES = 64 << 20
 
 
@@ -201,7 +204,7 @@ def problem_from_dict(d):
 
 
 def options(form="fp64", temporal_block=0, tile_cells=0, graph_launches=0,
-            device=-1, order="plain"):
+            device=-1, order="plain", generic=False):
     lib = load()
     o = mbh_options()
     lib.mbh_default_options(C.byref(o))
@@ -211,6 +214,7 @@ def options(form="fp64", temporal_block=0, tile_cells=0, graph_launches=0,
     o.graph_launches = graph_launches
     o.device = device
     o.step_order = ORDERS[order] if isinstance(order, str) else int(order)
+    o.flags = MBH_OPT_GENERIC if generic else 0
     return o
 
 

@@ -245,6 +245,7 @@ void bloch2_constants(const double* cq, double* o)
         o[22 + j] = D1[j];
     }
     o[26] = g;
+    o[27] = std::fma(o[7], o[7], 1.0); /* 1 + alpha0^2 at e = 0 */
 }
 
 /* reference (rho00, rho11, Re rho01, Im rho01) -> carried Bloch state
@@ -439,6 +440,10 @@ struct mbh_ctx {
     bool fp32 = false;
     bool zform = false;
     bool bloch = false; /* two-level Cayley in closed Bloch form */
+    /* every Bloch material has beta_y = 0 and a field-free alpha0
+     * (algo_bloch2r applies) */
+    bool bloch_real = false;
+    bool generic = false; /* MBH_OPT_GENERIC: no specialised kernels */
     std::vector<int32_t> mat_h; /* per super-grid cell (Bloch conversions) */
     std::vector<double> qm_h;   /* device blob copy (Bloch conversions) */
 
@@ -921,8 +926,11 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
 {
     uint32_t kform = kernel_form(ctx->algo, ctx->levels, ctx->form);
     if (ctx->algo == MBH_ALGO_NOOP && kform == MBH_FORM_MIXED &&
-        ctx->hz_zero) {
+        ctx->hz_zero && !ctx->generic) {
         kform = KFORM_NOOP_HZ0;
+    }
+    if (ctx->bloch && ctx->bloch_real && !ctx->generic) {
+        kform = ctx->fp32 ? KFORM_BLOCH2R_MIXED : KFORM_BLOCH2R_ZFORM64;
     }
     ctx->kform = kform;
     std::vector<uint32_t> cs;
@@ -1239,6 +1247,7 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     if (c->bloch) {
         c->DC = 4;
     }
+    c->generic = (opt.flags & MBH_OPT_GENERIC) != 0;
     const size_t esz = c->fp32 ? sizeof(float) : sizeof(double);
 
     /* super-grid layout */
@@ -1297,6 +1306,17 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
             }
         }
         moff += p.num_materials;
+    }
+
+    if (c->bloch) {
+        c->bloch_real = true;
+        for (size_t mi = 0; mi + BLOCH2_STRIDE <= qm.size();
+             mi += BLOCH2_STRIDE) {
+            const double* o = &qm[mi];
+            if (o[0] != 0.0 && (o[2] != 0.0 || o[5] != 0.0 || o[8] != 0.0)) {
+                c->bloch_real = false;
+            }
+        }
     }
 
     /* kernel and geometry */
@@ -1898,7 +1918,14 @@ int mbh_get_info(const mbh_ctx* ctx, mbh_info* info)
     info->graph_launches = ctx->graph_launches;
     info->step_order = ctx->order;
     info->mfma_tiles = (uint32_t)ctx->n_tiles_mfma;
-    info->flags = (ctx->kform == KFORM_NOOP_HZ0) ? MBH_INFO_HZ_ZERO : 0u;
+    info->flags = 0u;
+    if (ctx->kform == KFORM_NOOP_HZ0) {
+        info->flags |= MBH_INFO_HZ_ZERO;
+    }
+    if (ctx->kform == KFORM_BLOCH2R_MIXED ||
+        ctx->kform == KFORM_BLOCH2R_ZFORM64) {
+        info->flags |= MBH_INFO_BLOCH_REAL;
+    }
     return MBH_OK;
 }
 

@@ -373,9 +373,10 @@ struct algo_cayley {
  *   [10..13] F: D1^2 on (tau, m_z), row-major
  *   [14..17] p = P . (tau, m_x, m_y, m_z)
  *   [18..21] populations: rho00 = R00 tau + R01 m_z, rho11 = R10 tau + R11 m_z
- *   [22..25] D1 (row-major)  [26] g  [27] 0
+ *   [22..25] D1 (row-major)  [26] g  [27] 1 + alpha0^2 at e = 0
  * ==================================================================== */
-struct algo_bloch2 {
+template<bool REAL>
+struct algo_bloch2_t {
     static constexpr int STAGE = 0;
     static constexpr bool SHIFT = false;
     static constexpr int DC = 4;
@@ -402,6 +403,10 @@ struct algo_bloch2 {
     __device__ static void update_multi(cdp q, state* s, const double* e,
                                         double* p)
     {
+        if constexpr (REAL) {
+            update_real<C>(q, s, e, p);
+            return;
+        }
         const double bcx = q[1], bcy = q[2], bcz = q[3];
         const double bex = q[4], bey = q[5], bez = q[6];
         const double a0c = q[7], a0e = q[8], g2 = q[9];
@@ -439,6 +444,48 @@ struct algo_bloch2 {
         }
     }
 
+    /* The same step for a material whose beta_y and d alpha0 / d e are
+     * exactly zero (a diagonal Hamiltonian and a real dipole, as
+     * qm_desc_2lvl builds them, qm_description.hpp:338-432): the terms
+     * with beta_y are dropped and 1 + alpha0^2 is the blob's [27].  Each
+     * dropped term is an exact zero in the general form, so the results
+     * are the same (up to the sign of zeros). */
+    template<int C>
+    __device__ static void update_real(cdp q, state* s, const double* e,
+                                       double* p)
+    {
+        const double bcx = q[1], bcz = q[3];
+        const double bex = q[4], bez = q[6];
+        const double A0 = q[27], g2 = q[9];
+        const double F00 = q[10], F01 = q[11], F10 = q[12], F11 = q[13];
+        const double P0 = q[14], P1 = q[15], P2 = q[16], P3 = q[17];
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            double* v = s[c].v;
+            const double tr = fma(F01, v[3], F00 * v[0]);
+            const double mz = fma(F11, v[3], F10 * v[0]);
+            const double mx = g2 * v[1];
+            const double my = g2 * v[2];
+            const double bx = fma(e[c], bex, bcx);
+            const double bz = fma(e[c], bez, bcz);
+            const double r2 = fma(bz, bz, bx * bx);
+            const double sc = fma(-0.25, r2, A0);
+            const double lam = two_over(fma(sc, sc, r2));
+            const double c1x = -(bz * my);
+            const double c1y = fma(bz, mx, -(bx * mz));
+            const double c1z = bx * my;
+            const double bm = fma(bz, mz, bx * mx);
+            const double c2x = fma(bx, bm, -(mx * r2));
+            const double c2y = -(my * r2);
+            const double c2z = fma(bz, bm, -(mz * r2));
+            v[0] = tr;
+            v[1] = fma(lam, fma(sc, c1x, c2x), mx);
+            v[2] = fma(lam, fma(sc, c1y, c2y), my);
+            v[3] = fma(lam, fma(sc, c1z, c2z), mz);
+            p[c] = fma(P3, v[3], fma(P2, v[2], fma(P1, v[1], P0 * tr)));
+        }
+    }
+
     __device__ static void update(cdp q, state& s, double e, double& p)
     {
         update_multi<1>(q, &s, &e, &p);
@@ -459,6 +506,8 @@ struct algo_bloch2 {
         return population(q, s, 1) - population(q, s, 0);
     }
 };
+using algo_bloch2 = algo_bloch2_t<false>;
+using algo_bloch2r = algo_bloch2_t<true>;
 
 /* ---- coherence-vector population readout ------------------------------
  * cv_representation::calc_population

@@ -3,7 +3,9 @@
  * Reference operation order: FP64 storage implements MBH_FORM_FP64, FP32
  * storage MBH_FORM_MIXED (three-level Cayley; noop with the folded FDTD
  * constants).  Two-level Cayley in closed
- * Bloch form: FP32 storage MBH_FORM_MIXED, FP64 storage MBH_FORM_ZFORM64.
+ * Bloch form: FP32 storage MBH_FORM_MIXED, FP64 storage MBH_FORM_ZFORM64,
+ * each also specialised for a diagonal Hamiltonian and a real dipole
+ * (algo_bloch2r, chosen by the host from the constants).
  */
 #include "mbh_tile.hpp"
 
@@ -35,6 +37,14 @@ template __global__ void tile_kernel<algo_bloch2, 2, double>(KArgs);
 template __global__ void tile_kernel<algo_bloch2, 4, double>(KArgs);
 template __global__ void tile_kernel<algo_bloch2, 8, float>(KArgs);
 template __global__ void tile_kernel<algo_bloch2, 8, double>(KArgs);
+template __global__ void tile_kernel<algo_bloch2r, 1, float>(KArgs);
+template __global__ void tile_kernel<algo_bloch2r, 2, float>(KArgs);
+template __global__ void tile_kernel<algo_bloch2r, 4, float>(KArgs);
+template __global__ void tile_kernel<algo_bloch2r, 8, float>(KArgs);
+template __global__ void tile_kernel<algo_bloch2r, 1, double>(KArgs);
+template __global__ void tile_kernel<algo_bloch2r, 2, double>(KArgs);
+template __global__ void tile_kernel<algo_bloch2r, 4, double>(KArgs);
+template __global__ void tile_kernel<algo_bloch2r, 8, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, float>(KArgs);
 template __global__ void tile_kernel<algo_cayley<4>, 1, double>(KArgs);
@@ -70,6 +80,14 @@ const kernel_entry k_table_basic[] = {
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 4, algo_bloch2, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_MIXED, 8, algo_bloch2, float),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, MBH_FORM_ZFORM64, 8, algo_bloch2, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 1, algo_bloch2r, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 2, algo_bloch2r, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 4, algo_bloch2r, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 8, algo_bloch2r, float),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 1, algo_bloch2r, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 2, algo_bloch2r, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 4, algo_bloch2r, double),
+    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 8, algo_bloch2r, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_FP64, 1, algo_cayley<3>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_MIXED, 1, algo_cayley<3>, float),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 4, MBH_FORM_FP64, 1, algo_cayley<4>, double),

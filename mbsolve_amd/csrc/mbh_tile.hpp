@@ -95,8 +95,11 @@ struct KArgs {
     uint32_t n_mixed;
 };
 constexpr int32_t TILE_MAT_QM = 0x40000000;
-/* kernel-table form of algo_noop_hz0 (internal, beside the MBH_FORM_*) */
+/* kernel-table forms of the specialised kernels (internal, beside the
+ * MBH_FORM_*): algo_noop_hz0, and algo_bloch2r with FP32 / FP64 state */
 constexpr uint32_t KFORM_NOOP_HZ0 = 0x100;
+constexpr uint32_t KFORM_BLOCH2R_MIXED = 0x101;
+constexpr uint32_t KFORM_BLOCH2R_ZFORM64 = 0x102;
 
 template<typename ST>
 __device__ __forceinline__ double ld(const void* p, uint64_t i)
@@ -535,7 +538,11 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 template<class A, int C>
 constexpr int tile_min_waves()
 {
-    return (std::is_same<A, algo_bloch2>::value && C == 4) ? 3 : 1;
+    return ((std::is_same<A, algo_bloch2>::value ||
+             std::is_same<A, algo_bloch2r>::value) &&
+            C == 4)
+        ? 3
+        : 1;
 }
 
 template<class A, int C, typename ST, bool RED = false>

@@ -430,3 +430,39 @@ def test_graph_launches_auto(gpu):
         assert np.array_equal(a, b)
     for k in ("e", "h", "p", "d"):
         assert np.array_equal(s1[k], s0[k]), k
+
+
+@pytest.mark.parametrize("kind,form", [("sit2", "mixed"), ("sit2", "zform64"),
+                                       ("maxwell", "mixed")])
+def test_specialised_kernels_match_general(gpu, kind, form):
+    """The kernels chosen from the constants (two-level Bloch form with a
+    diagonal Hamiltonian and a real dipole; pure Maxwell with the held H
+    cells starting at zero) drop only terms that are exact zeros: records
+    and state equal those of the general kernels (MBH_OPT_GENERIC)."""
+    if kind == "sit2":
+        spec = setups.sit_2lvl(num_gridpoints=1500, endtime=0.4e-12,
+                               num_records=12)
+        flag = capi.MBH_INFO_BLOCH_REAL
+    else:
+        spec = setups.maxwell_line(num_gridpoints=3000, num_timesteps=700)
+        spec["records"] = [{"name": "e", "interval": spec["endtime"] / 8}]
+        flag = capi.MBH_INFO_HZ_ZERO
+    dev, scen = setups.build(spec)
+    suffix = "zform" if form == "zform64" else form
+    prob = mb.build_problem(setups.solver_name(spec, suffix), dev, scen)
+    nt = int(prob["num_timesteps"])
+    out = []
+    for generic in (False, True):
+        print("case specialised", kind, form, "generic", generic, flush=True)
+        ctx = capi.Context([prob], capi.options(form, temporal_block=8,
+                                                generic=generic))
+        flags = ctx.info()["flags"]
+        assert bool(flags & flag) == (not generic), flags
+        ctx.run(0, nt)
+        out.append((ctx.records(0), ctx.state(0)))
+        ctx.close()
+    (r0, s0), (r1, s1) = out
+    assert np.max(np.abs(r0)) > 0
+    assert np.array_equal(r0, r1)
+    for k in ("e", "h", "p", "d"):
+        assert np.array_equal(s0[k], s1[k]), k
