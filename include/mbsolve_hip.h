@@ -250,8 +250,9 @@ typedef struct mbh_options {
 
 /* mbh_options.flags: run the general kernels of the form even where the
  * problem admits a specialised one (pure Maxwell with every held H cell
- * starting at zero; two-level Bloch form with a diagonal Hamiltonian and a
- * real dipole).  The specialised kernels drop terms that are exact zeros,
+ * starting at zero; two-level Bloch form with a diagonal Hamiltonian, a
+ * real off-diagonal dipole and lossless materials, held H starting at
+ * zero).  The specialised kernels drop terms that are exact zeros,
  * so the results are the same; this flag is for verifying that. */
 #define MBH_OPT_GENERIC 1u
 
@@ -274,8 +275,9 @@ typedef struct mbh_info {
 /* mbh_info.flags: the pure-Maxwell FP32 kernel without the H factor of
  * the cells held at zero is in use (every such cell starts with H = 0) */
 #define MBH_INFO_HZ_ZERO 1u
-/* mbh_info.flags: the two-level Bloch kernel specialised for beta_y = 0 and
- * a field-free alpha0 (diagonal Hamiltonian, real dipole) is in use */
+/* mbh_info.flags: the two-level Bloch kernel specialised for a diagonal
+ * Hamiltonian, a real off-diagonal dipole and lossless materials, with
+ * every held H cell starting at zero, is in use */
 #define MBH_INFO_BLOCH_REAL 2u
 
 typedef struct mbh_ctx mbh_ctx;

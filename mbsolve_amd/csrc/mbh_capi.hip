@@ -440,8 +440,9 @@ struct mbh_ctx {
     bool fp32 = false;
     bool zform = false;
     bool bloch = false; /* two-level Cayley in closed Bloch form */
-    /* every Bloch material has beta_y = 0 and a field-free alpha0
-     * (algo_bloch2r applies) */
+    /* every Bloch material has beta_y = 0, a field-free alpha0, no p
+     * term in tau and m_z, and no loss (with hz_zero, algo_bloch2r
+     * applies) */
     bool bloch_real = false;
     bool generic = false; /* MBH_OPT_GENERIC: no specialised kernels */
     std::vector<int32_t> mat_h; /* per super-grid cell (Bloch conversions) */
@@ -929,7 +930,7 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
         ctx->hz_zero && !ctx->generic) {
         kform = KFORM_NOOP_HZ0;
     }
-    if (ctx->bloch && ctx->bloch_real && !ctx->generic) {
+    if (ctx->bloch && ctx->bloch_real && ctx->hz_zero && !ctx->generic) {
         kform = ctx->fp32 ? KFORM_BLOCH2R_MIXED : KFORM_BLOCH2R_ZFORM64;
     }
     ctx->kform = kform;
@@ -1309,11 +1310,20 @@ int mbh_create(const mbh_problem* runs, uint32_t num_runs,
     }
 
     if (c->bloch) {
+        /* algo_bloch2r: beta_y, d alpha0 / d e and the p coefficients of
+         * tau and m_z exactly zero in every material with a density, and
+         * every material lossless (fac_a exactly 1) */
         c->bloch_real = true;
         for (size_t mi = 0; mi + BLOCH2_STRIDE <= qm.size();
              mi += BLOCH2_STRIDE) {
             const double* o = &qm[mi];
-            if (o[0] != 0.0 && (o[2] != 0.0 || o[5] != 0.0 || o[8] != 0.0)) {
+            if (o[0] != 0.0 && (o[2] != 0.0 || o[5] != 0.0 || o[8] != 0.0 ||
+                                o[14] != 0.0 || o[17] != 0.0)) {
+                c->bloch_real = false;
+            }
+        }
+        for (size_t mi = 0; mi < fdtd.size(); mi += FDTD_STRIDE) {
+            if (fdtd[mi] != 1.0) {
                 c->bloch_real = false;
             }
         }

@@ -444,12 +444,13 @@ struct algo_bloch2_t {
         }
     }
 
-    /* The same step for a material whose beta_y and d alpha0 / d e are
-     * exactly zero (a diagonal Hamiltonian and a real dipole, as
-     * qm_desc_2lvl builds them, qm_description.hpp:338-432): the terms
-     * with beta_y are dropped and 1 + alpha0^2 is the blob's [27].  Each
-     * dropped term is an exact zero in the general form, so the results
-     * are the same (up to the sign of zeros). */
+    /* The same step for a material whose beta_y, d alpha0 / d e and the
+     * p coefficients of tau and m_z are exactly zero (a diagonal
+     * Hamiltonian and a real off-diagonal dipole, as qm_desc_2lvl builds
+     * them, qm_description.hpp:338-432): those terms are dropped and
+     * 1 + alpha0^2 is the blob's [27].  Each dropped term is an exact zero
+     * in the general form, so the results are the same (up to the sign of
+     * zeros). */
     template<int C>
     __device__ static void update_real(cdp q, state* s, const double* e,
                                        double* p)
@@ -458,7 +459,7 @@ struct algo_bloch2_t {
         const double bex = q[4], bez = q[6];
         const double A0 = q[27], g2 = q[9];
         const double F00 = q[10], F01 = q[11], F10 = q[12], F11 = q[13];
-        const double P0 = q[14], P1 = q[15], P2 = q[16], P3 = q[17];
+        const double P1 = q[15], P2 = q[16];
 #pragma unroll
         for (int c = 0; c < C; c++) {
             double* v = s[c].v;
@@ -482,7 +483,7 @@ struct algo_bloch2_t {
             v[1] = fma(lam, fma(sc, c1x, c2x), mx);
             v[2] = fma(lam, fma(sc, c1y, c2y), my);
             v[3] = fma(lam, fma(sc, c1z, c2z), mz);
-            p[c] = fma(P3, v[3], fma(P2, v[2], fma(P1, v[1], P0 * tr)));
+            p[c] = fma(P2, v[2], P1 * v[1]);
         }
     }
 

@@ -432,6 +432,10 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
             const double hn = (c + 1 < C) ? h[(c + 1) % C] : h_right;
             if constexpr (A::FAST_FDTD && A::DC == 0) {
                 en[c] = fma(fx[c], hn - h[c], fa[c] * e[c]); /* p = 0 */
+            } else if constexpr (std::is_same<A, algo_bloch2r>::value) {
+                /* lossless (fa = 1 in the grid; cells outside it and the
+                 * pseudo cells hold E = 0 with fg = fx = 0) */
+                en[c] = fma(fx[c], hn - h[c], fma(fg[c], p[c], e[c]));
             } else if constexpr (A::FAST_FDTD) {
                 en[c] = fma(fx[c], hn - h[c], fma(fg[c], p[c], fa[c] * e[c]));
             } else {
@@ -465,7 +469,9 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 #pragma unroll
         for (int c = 0; c < C; c++) {
             const double ep = (c > 0) ? en[(c + C - 1) % C] : e_left;
-            if constexpr (std::is_same<A, algo_noop_hz0>::value) {
+            if constexpr (std::is_same<A, algo_noop_hz0>::value ||
+                          std::is_same<A, algo_bloch2r>::value) {
+                /* every held cell starts at zero and its fc is zero */
                 h[c] = fma(fc[c], en[c] - ep, h[c]);
             } else if constexpr (A::FAST_FDTD) {
                 h[c] = fma(fc[c], en[c] - ep, fb[c] * h[c]);
@@ -535,14 +541,15 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 /* wavefronts per SIMD the compiler is asked to fit (1: no request): the
  * two-level Bloch kernel at four cells per lane fits three without spills
  * in its step loop, against two unasked */
+#ifndef MBH_BLOCH2R_WAVES
+#define MBH_BLOCH2R_WAVES 3
+#endif
 template<class A, int C>
 constexpr int tile_min_waves()
 {
-    return ((std::is_same<A, algo_bloch2>::value ||
-             std::is_same<A, algo_bloch2r>::value) &&
-            C == 4)
-        ? 3
-        : 1;
+    return (std::is_same<A, algo_bloch2r>::value && C == 4)
+        ? MBH_BLOCH2R_WAVES
+        : ((std::is_same<A, algo_bloch2>::value && C == 4) ? 3 : 1);
 }
 
 template<class A, int C, typename ST, bool RED = false>
