@@ -466,3 +466,30 @@ def test_specialised_kernels_match_general(gpu, kind, form):
     assert np.array_equal(r0, r1)
     for k in ("e", "h", "p", "d"):
         assert np.array_equal(s0[k], s1[k]), k
+
+
+@pytest.mark.parametrize("form", ["mixed", "zform"])
+def test_bloch_forms_independent_of_tile_width(gpu, form):
+    """The two-level Bloch forms at 1, 2 and 4 cells per lane (64-, 128-
+    and 256-cell tiles) and one temporal block: the same records and state
+    bit for bit (per-cell arithmetic does not depend on the tiling; with
+    FP32 state the launch boundaries, fixed by the block, are the same)."""
+    spec = setups.sit_2lvl(num_gridpoints=3000, endtime=0.4e-12,
+                           num_records=12)
+    out = []
+    for tile in (64, 128, 256):
+        dev, scen = setups.build(spec)
+        name = setups.solver_name(spec, form)
+        sol = mb.create_hip_solver(name, dev, scen, temporal_block=16,
+                                   tile_cells=tile)
+        info = sol.info()
+        print("case tile", form, tile, info, flush=True)
+        assert info["tile_cells"] == tile
+        sol.run()
+        out.append(([r.get_data_real() for r in sol.get_results()],
+                    sol.fetch_state()))
+    for recs, st in out[1:]:
+        for a, b in zip(recs, out[0][0]):
+            assert np.array_equal(a, b)
+        for k in ("e", "h", "p", "d"):
+            assert np.array_equal(st[k], out[0][1][k]), k
