@@ -935,7 +935,9 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
     }
     ctx->kform = kform;
     std::vector<uint32_t> cs;
-    for (uint32_t C : { 1u, 2u, 4u, 8u }) {
+    /* cells per lane (tiles of several wavefronts count as their cells
+     * per lane of the first) */
+    for (uint32_t C : { 1u, 2u, 4u, 8u, 16u }) {
         if (find_kernel(ctx->algo, ctx->levels, kform, C, ctx->order) &&
             (!req_tile || req_tile == 64u * C)) {
             cs.push_back(C);

@@ -5,7 +5,8 @@
  * constants).  Two-level Cayley in closed
  * Bloch form: FP32 storage MBH_FORM_MIXED, FP64 storage MBH_FORM_ZFORM64,
  * each also specialised for a diagonal Hamiltonian and a real dipole
- * (algo_bloch2r, chosen by the host from the constants).
+ * (algo_bloch2r, chosen by the host from the constants), the latter also
+ * as tiles of 2 and 4 wavefronts at 4 cells per lane.
  */
 #include "mbh_tile.hpp"
 
@@ -13,6 +14,9 @@ namespace mbh {
 
 #define MBH_ENTRY(ALGO_ID, LEV, FORM, C, A, ST)                               \
     kernel_entry { ALGO_ID, LEV, FORM, C, &launch_tile<A, C, ST> }
+/* a tile of W wavefronts at C cells per lane: listed as C W cells per lane */
+#define MBH_ENTRY_MW(ALGO_ID, LEV, FORM, C, W, A, ST)                         \
+    kernel_entry { ALGO_ID, LEV, FORM, (C) * (W), &launch_tile_mw<A, C, W, ST> }
 
 template __global__ void tile_kernel<algo_noop, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_noop, 2, double>(KArgs);
@@ -40,11 +44,13 @@ template __global__ void tile_kernel<algo_bloch2, 8, double>(KArgs);
 template __global__ void tile_kernel<algo_bloch2r, 1, float>(KArgs);
 template __global__ void tile_kernel<algo_bloch2r, 2, float>(KArgs);
 template __global__ void tile_kernel<algo_bloch2r, 4, float>(KArgs);
-template __global__ void tile_kernel<algo_bloch2r, 8, float>(KArgs);
+template __global__ void tile_kernel_mw<algo_bloch2r, 4, 2, float>(KArgs);
+template __global__ void tile_kernel_mw<algo_bloch2r, 4, 4, float>(KArgs);
 template __global__ void tile_kernel<algo_bloch2r, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_bloch2r, 2, double>(KArgs);
 template __global__ void tile_kernel<algo_bloch2r, 4, double>(KArgs);
-template __global__ void tile_kernel<algo_bloch2r, 8, double>(KArgs);
+template __global__ void tile_kernel_mw<algo_bloch2r, 4, 2, double>(KArgs);
+template __global__ void tile_kernel_mw<algo_bloch2r, 4, 4, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, double>(KArgs);
 template __global__ void tile_kernel<algo_cayley<3>, 1, float>(KArgs);
 template __global__ void tile_kernel<algo_cayley<4>, 1, double>(KArgs);
@@ -83,11 +89,13 @@ const kernel_entry k_table_basic[] = {
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 1, algo_bloch2r, float),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 2, algo_bloch2r, float),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 4, algo_bloch2r, float),
-    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 8, algo_bloch2r, float),
+    MBH_ENTRY_MW(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 4, 2, algo_bloch2r, float),
+    MBH_ENTRY_MW(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_MIXED, 4, 4, algo_bloch2r, float),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 1, algo_bloch2r, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 2, algo_bloch2r, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 4, algo_bloch2r, double),
-    MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 8, algo_bloch2r, double),
+    MBH_ENTRY_MW(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 4, 2, algo_bloch2r, double),
+    MBH_ENTRY_MW(MBH_ALGO_REG_CAYLEY, 2, KFORM_BLOCH2R_ZFORM64, 4, 4, algo_bloch2r, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_FP64, 1, algo_cayley<3>, double),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 3, MBH_FORM_MIXED, 1, algo_cayley<3>, float),
     MBH_ENTRY(MBH_ALGO_REG_CAYLEY, 4, MBH_FORM_FP64, 1, algo_cayley<4>, double),

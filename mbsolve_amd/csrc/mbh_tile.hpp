@@ -235,18 +235,31 @@ __device__ __forceinline__ void add_shift(const KArgs& k, int32_t m,
 constexpr int TILE_SOURCES = 8;
 constexpr int MAX_BLOCK = 64; /* largest temporal block */
 
-/* One tile t: all steps of the temporal block for the tile's cells. */
-template<class A, int C, typename ST, bool RED = false>
+/* One tile t: all steps of the temporal block for the tile's cells.
+ * With W > 1 the tile spans W wavefronts of one workgroup, each holding a
+ * contiguous run of 64 C cells; at the inner edges between them the
+ * neighbour's boundary H (before the E update) and E (before the H
+ * update) come through LDS every step, so only the tile's two outer edges
+ * carry a halo (64 C W - 2 tb owned cells instead of 64 C - 2 tb). */
+template<class A, int C, typename ST, bool RED = false, int W = 1>
 __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
 {
+    static_assert(W == 1 || !RED, "multi-wavefront tiles: plain order only");
+    static_assert(W == 1 || A::STAGE == 0,
+                  "multi-wavefront tiles: no per-wavefront LDS staging");
     __shared__ double s_src[TILE_SOURCES][MAX_BLOCK];
+    /* boundary values of each wavefront's run for its neighbours */
+    __shared__ double x_h[W > 1 ? W : 1]; /* H of the first cell */
+    __shared__ double x_e[W > 1 ? W : 1]; /* E of the last cell */
 
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wv = (W > 1) ? (int)(threadIdx.x >> 6) : 0;
     const int64_t own_lo = (int64_t)(t * k.owned);
     const int64_t own_hi =
         ((own_lo + (int64_t)k.owned) < (int64_t)k.Gown) ? own_lo + k.owned
                                                          : (int64_t)k.Gown;
-    const int64_t gl = own_lo - (int64_t)k.halo + (int64_t)lane * C;
+    const int64_t gl =
+        own_lo - (int64_t)k.halo + (int64_t)(wv * 64 + lane) * C;
     const uint64_t n0 = *as_const(k.n_base) + k.n_off;
     const uint32_t nsteps = k.nsteps;
 
@@ -357,9 +370,17 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
     const uint32_t nsrc = s_hi - s_lo;
     const bool src_staged = nsrc <= TILE_SOURCES && nsteps <= MAX_BLOCK;
     if (src_staged && nsrc > 0) {
-        for (uint32_t j = 0; j < nsrc; j++) {
-            const double* data = as_const(k.src)[s_lo + j].data;
-            s_src[j][lane] = (lane < (int)nsteps) ? data[n0 + lane] : 0.0;
+        if (wv == 0) {
+            for (uint32_t j = 0; j < nsrc; j++) {
+                const double* data = as_const(k.src)[s_lo + j].data;
+                s_src[j][lane] = (lane < (int)nsteps) ? data[n0 + lane] : 0.0;
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (W > 1) {
+        if (lane == 0) {
+            x_h[wv] = h[0];
         }
         __syncthreads();
     }
@@ -425,7 +446,13 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
         }
 
         /* E update (solver_cpu_fdtd.tpp:238-243) */
-        const double h_right = from_right(h[0]);
+        double h_right = from_right(h[0]);
+        if constexpr (W > 1) {
+            /* the last lane's right neighbour is the next run's first
+             * cell (beyond the tile's last run: halo, as with W = 1) */
+            const double xr = (wv + 1 < W) ? x_h[wv + 1] : 0.0;
+            h_right = (lane == 63) ? xr : h_right;
+        }
         double en[C];
 #pragma unroll
         for (int c = 0; c < C; c++) {
@@ -465,7 +492,15 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
         }
 
         /* H update and boundary condition (solver_cpu_fdtd.tpp:263-275) */
-        const double e_left = from_left(en[C - 1]);
+        double e_left = from_left(en[C - 1]);
+        if constexpr (W > 1) {
+            if (lane == 63) {
+                x_e[wv] = en[C - 1];
+            }
+            __syncthreads();
+            const double xl = (wv > 0) ? x_e[wv - 1] : 0.0;
+            e_left = (lane == 0) ? xl : e_left;
+        }
 #pragma unroll
         for (int c = 0; c < C; c++) {
             const double ep = (c > 0) ? en[(c + C - 1) % C] : e_left;
@@ -478,6 +513,14 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
             } else {
                 h[c] = hz[c] ? 0.0 : h[c] + fc[c] * (en[c] - ep);
             }
+        }
+        if constexpr (W > 1) {
+            /* the next step's E update reads it (every read of the old
+             * value came before the barrier above) */
+            if (lane == 0) {
+                x_h[wv] = h[0];
+            }
+            __syncthreads();
         }
 
 #pragma unroll
@@ -561,6 +604,16 @@ tile_kernel(KArgs k)
                                             : blockIdx.x);
 }
 
+/* tile of W wavefronts (64 W threads) */
+template<class A, int C, int W, typename ST>
+__global__ __launch_bounds__(64 * W)
+__attribute__((amdgpu_waves_per_eu(tile_min_waves<A, C>()))) void
+tile_kernel_mw(KArgs k)
+{
+    tile_body<A, C, ST, false, W>(k, k.tile_list ? k.tile_list[blockIdx.x]
+                                                 : blockIdx.x);
+}
+
 /* ---- launcher registry ------------------------------------------------- */
 typedef hipError_t (*tile_launch_fn)(const KArgs& k, uint64_t num_tiles,
                                      hipStream_t stream);
@@ -579,6 +632,18 @@ hipError_t launch_tile(const KArgs& k, uint64_t num_tiles, hipStream_t stream)
 {
     hipLaunchKernelGGL((tile_kernel<A, C, ST, RED>), dim3((unsigned)num_tiles),
                        dim3(64), 0, stream, k);
+    return hipGetLastError();
+}
+
+/* multi-wavefront tiles: the tables list them with C W cells per lane of
+ * the tile's first wavefront, so tile geometry (64 C W cells) needs no
+ * other change on the host */
+template<class A, int C, int W, typename ST>
+hipError_t launch_tile_mw(const KArgs& k, uint64_t num_tiles,
+                          hipStream_t stream)
+{
+    hipLaunchKernelGGL((tile_kernel_mw<A, C, W, ST>),
+                       dim3((unsigned)num_tiles), dim3(64 * W), 0, stream, k);
     return hipGetLastError();
 }
 

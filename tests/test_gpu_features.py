@@ -471,13 +471,15 @@ def test_specialised_kernels_match_general(gpu, kind, form):
 @pytest.mark.parametrize("form", ["mixed", "zform"])
 def test_bloch_forms_independent_of_tile_width(gpu, form):
     """The two-level Bloch forms at 1, 2 and 4 cells per lane (64-, 128-
-    and 256-cell tiles) and one temporal block: the same records and state
-    bit for bit (per-cell arithmetic does not depend on the tiling; with
-    FP32 state the launch boundaries, fixed by the block, are the same)."""
+    and 256-cell tiles) and in tiles of 2 and 4 wavefronts (512 and 1024
+    cells, inner edges exchanged through LDS) at one temporal
+    block: the same records and state bit for bit (per-cell arithmetic does
+    not depend on the tiling; with FP32 state the launch boundaries, fixed
+    by the block, are the same)."""
     spec = setups.sit_2lvl(num_gridpoints=3000, endtime=0.4e-12,
                            num_records=12)
     out = []
-    for tile in (64, 128, 256):
+    for tile in (64, 128, 256, 512, 1024):
         dev, scen = setups.build(spec)
         name = setups.solver_name(spec, form)
         sol = mb.create_hip_solver(name, dev, scen, temporal_block=16,
@@ -493,3 +495,32 @@ def test_bloch_forms_independent_of_tile_width(gpu, form):
             assert np.array_equal(a, b)
         for k in ("e", "h", "p", "d"):
             assert np.array_equal(st[k], out[0][1][k]), k
+
+
+@pytest.mark.parametrize("tile", [512, 1024])
+def test_multi_wavefront_tiles_in_a_batch(gpu, tile):
+    """Five two-level runs of uneven length in one context (run boundaries
+    and the sources at each run's first cell fall inside tiles and at
+    wavefront edges): tiles of several wavefronts give the records of the
+    256-cell tiles bit for bit, FP64 state, temporal block 24."""
+    dx = 150e-6 / 8191
+    specs = [setups.sit_2lvl(num_gridpoints=n, endtime=0.3e-12,
+                             num_records=8, length=(n - 1) * dx,
+                             area=a, name="w%d" % i)
+             for i, (n, a) in enumerate(((700, 6.0), (1300, 3.0), (257, 9.0),
+                                         (2100, 6.3), (999, 4.0)))]
+    out = []
+    for tc in (256, tile):
+        mb.material.clear_library()
+        runs = [setups.build(sp, clear_library=False) for sp in specs]
+        print("case batch tile", tc, flush=True)
+        b = mb.solver_hip_batch("hip-fdtd-2lvl-reg-cayley-zform", runs,
+                                temporal_block=24, tile_cells=tc)
+        assert b.info()["tile_cells"] == tc
+        b.run()
+        out.append([[x.get_data_real() for x in b.get_results(r)]
+                    for r in range(len(specs))])
+    for r in range(len(specs)):
+        for a, c in zip(out[0][r], out[1][r]):
+            assert np.max(np.abs(a)) > 0
+            assert np.array_equal(a, c), (tile, r)
