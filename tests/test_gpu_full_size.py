@@ -9,6 +9,8 @@ digitized curves.
   in seconds), FP32 storage and FP64, each run against its own oracle.
 * song2005 (mbsolve-tool.cpp:163-248) and marskar2011 (mbsolve-tool.cpp:
   479-662) through the GPU forms against tools/reference-data.
+* The six-level gain medium over a bench pass's 20000 steps (on 4096 of
+  its cells) in the z-form and the MFMA form.
 """
 import os
 
@@ -164,3 +166,34 @@ def test_marskar2011_reference_curves_gpu(gpu, form, tol):
         assert err <= tol, (form, r.get_name(), err)
     dt = spec["endtime"] / (sol.problem()["num_timesteps"] - 1)
     check_marskar_curves(recs, dt)
+
+
+@pytest.fixture(scope="module")
+def six_level_pass():
+    """The oracle's records of the six-level gain medium over the 20000
+    steps of a bench pass on 4096 cells (computed once for the module)."""
+    spec = setups.qcl6(num_gridpoints=4096, num_timesteps=20000,
+                       num_records=16)
+    dev, scen = setups.build(spec)
+    prob = mb.build_problem(setups.solver_name(spec), dev, scen)
+    print("oracle six-level 4096 x 20000", flush=True)
+    return spec, oracle.OracleRun(prob).run(threads=THREADS).records()
+
+
+@pytest.mark.parametrize("form,tol", [("zform", 1e-11), ("mfma64", 1e-6)])
+def test_six_level_over_the_bench_pass_length(gpu, six_level_pass, form,
+                                              tol):
+    """The six-level gain medium over the 20000 steps of a bench pass on
+    4096 cells (the workload's dx, so its dt) against the oracle: the
+    FP64 z-form within 1e-11, the MFMA form within 1e-6 (inversion and
+    populations absolute, E relative to its maximum)."""
+    spec, ref = six_level_pass
+    sol, recs = _hip(spec, form)
+    for r, a, b in zip(sol.get_results(), recs, ref):
+        if r.get_name().startswith(("d", "inv")):
+            err = float(np.max(np.abs(a - b)))
+        else:
+            err = rel_err(a, b)
+        print("  ", form, r.get_name(), err, flush=True)
+        assert np.max(np.abs(b)) > 0
+        assert err <= tol, (form, r.get_name(), err)
