@@ -218,16 +218,19 @@ template<class A>
 __device__ __forceinline__ void add_shift(const KArgs& k, int32_t m,
                                           double* v, double sign)
 {
-    if (m < 0) {
-        return;
-    }
-    const double* q = (const double*)k.qm + (uint64_t)m * k.qstride;
-    if (q[0] == 0.0) {
-        return;
-    }
+    /* loads from material 0's blob where m < 0 (every blob is there), so
+     * that none waits in a branch of its own */
+    const double* q =
+        (const double*)k.qm + (uint64_t)(m < 0 ? 0 : m) * k.qstride;
+    double y[A::DC];
 #pragma unroll
     for (int j = 0; j < A::DC; j++) {
-        v[j] = fma(sign, q[A::OFF_YEQ + j], v[j]);
+        y[j] = q[A::OFF_YEQ + j];
+    }
+    const bool on = (m >= 0) && (q[0] != 0.0);
+#pragma unroll
+    for (int j = 0; j < A::DC; j++) {
+        v[j] = on ? fma(sign, y[j], v[j]) : v[j];
     }
 }
 
@@ -269,41 +272,57 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
     int32_t m[C];
     bool hz[C];
 
+    /* The state loads of all cells first, each from a clamped address
+     * (cell 0 where the cell lies outside the grid) and selected after,
+     * then the loads that depend on the cells' materials: no load waits
+     * in a branch of its own, and each group is in flight together. */
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const int64_t g = gl + c;
         const bool in = (g >= 0) && (g < (int64_t)k.G);
-        m[c] = in ? k.mat[g] : -1;
-        hz[c] = in ? (k.hzero[g] != 0) : true;
-        e[c] = in ? ld<ST>(k.e_in, g) : 0.0;
+        const uint64_t gc = in ? (uint64_t)g : 0u;
+        const int32_t mv = k.mat[gc];
+        const uint8_t hv = k.hzero[gc];
+        const double ev = ld<ST>(k.e_in, gc);
+        const double hv2 = ld<ST>(k.h_in, gc);
+        const double pv = (A::DC > 0) ? ld<ST>(k.p_in, gc) : 0.0;
+        double dv[A::DC > 0 ? A::DC : 1];
+#pragma unroll
+        for (int j = 0; j < A::DC; j++) {
+            dv[j] = ld<ST>(k.d_in, (uint64_t)j * k.dstride + gc);
+        }
+        m[c] = in ? mv : -1;
+        hz[c] = in ? (hv != 0) : true;
+        e[c] = in ? ev : 0.0;
         /* the trailing pseudo cell is never owned, so never written back:
          * its H is h[N] of the initial state at step 0 and zero after (the
          * boundary condition) */
-        h[c] = (in && (g + 1 < (int64_t)k.G || n0 == 0))
-            ? ld<ST>(k.h_in, g)
-            : 0.0;
+        h[c] = (in && (g + 1 < (int64_t)k.G || n0 == 0)) ? hv2 : 0.0;
         /* no density: p keeps its initial zero (lindblad_noop) and is
          * neither read nor written */
-        p[c] = (A::DC > 0 && in) ? ld<ST>(k.p_in, g) : 0.0;
+        p[c] = (A::DC > 0 && in) ? pv : 0.0;
 #pragma unroll
         for (int j = 0; j < A::DC; j++) {
-            d[c].v[j] = in ? ld<ST>(k.d_in, (uint64_t)j * k.dstride + g) : 0.0;
+            d[c].v[j] = in ? dv[j] : 0.0;
         }
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) {
         if constexpr (A::SHIFT && std::is_same<ST, float>::value) {
             if (k.yshift) {
                 add_shift<A>(k, m[c], d[c].v, 1.0);
             }
         }
-        if (m[c] >= 0) {
-            const double* f = k.fdtd + (uint64_t)m[c] * FDTD_STRIDE;
-            fa[c] = f[0];
-            fb[c] = f[1];
-            fc[c] = f[2];
-            fg[c] = f[3];
-            fx[c] = f[4];
-        } else {
-            fa[c] = fb[c] = fc[c] = fg[c] = fx[c] = 0.0;
-        }
+        /* material 0's row where the cell has none, then zeroed */
+        const double* f =
+            k.fdtd + (uint64_t)(m[c] < 0 ? 0 : m[c]) * FDTD_STRIDE;
+        const double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3], f4 = f[4];
+        const bool known = m[c] >= 0;
+        fa[c] = known ? f0 : 0.0;
+        fb[c] = known ? f1 : 0.0;
+        fc[c] = known ? f2 : 0.0;
+        fg[c] = known ? f3 : 0.0;
+        fx[c] = known ? f4 : 0.0;
         if constexpr (A::FAST_FDTD) {
             /* folded constants: e' = fa e + fg' p + fx' (h_r - h) with
              * fg' = -fb fg, fx' = fb fx (zero outside the grid, so those
@@ -329,9 +348,10 @@ __device__ __forceinline__ void tile_body(const KArgs& k, uint64_t t)
     for (int c = 0; c < C; c++) {
         bool act = false;
         if constexpr (A::DC > 0) {
-            if (m[c] >= 0) {
-                act = A::has_qm((cdp)k.qm + (uint64_t)m[c] * k.qstride);
-            }
+            /* material 0's flag read where the cell has none */
+            const bool q = A::has_qm(
+                (cdp)k.qm + (uint64_t)(m[c] < 0 ? 0 : m[c]) * k.qstride);
+            act = (m[c] >= 0) && q;
         }
         ma[c] = act ? m[c] : -1;
     }

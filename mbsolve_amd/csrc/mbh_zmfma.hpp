@@ -338,12 +338,18 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
 
     /* FDTD state, one cell per lane (folded constants as in tile_kernel) */
     const bool in = (gl >= 0) && (gl < (int64_t)k.G);
-    /* a uniform tile lies inside the grid: every lane has the material */
+    /* a uniform tile lies inside the grid: every lane has the material
+     * (the loads go to a clamped address, without a branch of their own) */
     const int32_t m = tm & ~TILE_MAT_QM;
-    const bool hz = in ? (k.hzero[gl] != 0) : true;
-    double e = in ? ld<ST>(k.e_in, gl) : 0.0;
-    double h = in ? ld<ST>(k.h_in, gl) : 0.0;
-    double p = in ? ld<ST>(k.p_in, gl) : 0.0;
+    const uint64_t glc = in ? (uint64_t)gl : 0u;
+    const uint8_t hzv = k.hzero[glc];
+    const double e0 = ld<ST>(k.e_in, glc);
+    const double h0 = ld<ST>(k.h_in, glc);
+    const double p0 = ld<ST>(k.p_in, glc);
+    const bool hz = in ? (hzv != 0) : true;
+    double e = in ? e0 : 0.0;
+    double h = in ? h0 : 0.0;
+    double p = in ? p0 : 0.0;
     double fa = 0.0, fg = 0.0, fx = 0.0, fc = 0.0;
     if (m >= 0) {
         const double* f = k.fdtd + (uint64_t)m * FDTD_STRIDE;
@@ -354,20 +360,24 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
     }
     const double hm = hz ? 0.0 : 1.0;
 
-    /* density state, MFMA layout */
+    /* density state, MFMA layout.  The loads go to clamped addresses
+     * (padding slots read row K - 1) and the padding is zeroed after, so
+     * that no load sits in a branch of its own and all of them are in
+     * flight together. */
     double Y[Gm::NB][Gm::ROWS];
     int32_t mb[Gm::NB];
 #pragma unroll
     for (int n = 0; n < Gm::NB; n++) {
         const int64_t gn = g0 + 16 * n + j;
         const bool inn = (gn >= 0) && (gn < (int64_t)k.G);
+        const uint64_t gc = inn ? (uint64_t)gn : 0u;
         mb[n] = m;
 #pragma unroll
         for (int u = 0; u < Gm::ROWS; u++) {
             const int row = Gm::slot_row(q, u);
-            Y[n][u] = (inn && row < Gm::K)
-                ? ld<ST>(k.d_in, (uint64_t)row * k.dstride + gn)
-                : 0.0;
+            const int rc = row < Gm::K ? row : Gm::K - 1;
+            const double v = ld<ST>(k.d_in, (uint64_t)rc * k.dstride + gc);
+            Y[n][u] = (inn && row < Gm::K) ? v : 0.0;
         }
     }
 
@@ -383,14 +393,20 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
     const bool shifted =
         std::is_same<ST, float>::value && k.yshift && q0_active;
     if (shifted) {
+        /* y* of this lane's slots (the same rows in every block), loaded
+         * unconditionally as above */
+        double ys[Gm::ROWS];
+#pragma unroll
+        for (int u = 0; u < Gm::ROWS; u++) {
+            const int row = Gm::slot_row(q, u);
+            ys[u] = ystar[row < Gm::K ? row : Gm::K - 1];
+        }
 #pragma unroll
         for (int n = 0; n < Gm::NB; n++) {
 #pragma unroll
             for (int u = 0; u < Gm::ROWS; u++) {
                 const int row = Gm::slot_row(q, u);
-                if (row < Gm::K) {
-                    Y[n][u] = fma(1.0, ystar[row], Y[n][u]);
-                }
+                Y[n][u] = (row < Gm::K) ? fma(1.0, ys[u], Y[n][u]) : Y[n][u];
             }
         }
     }
@@ -572,6 +588,13 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
         st<ST>(k.h_out, gl, h);
         st<ST>(k.p_out, gl, p);
     }
+    /* y* again (unconditional loads, then the stores) */
+    double ys[Gm::ROWS];
+#pragma unroll
+    for (int u = 0; u < Gm::ROWS; u++) {
+        const int row = Gm::slot_row(q, u);
+        ys[u] = shifted ? ystar[row < Gm::K ? row : Gm::K - 1] : 0.0;
+    }
 #pragma unroll
     for (int n = 0; n < Gm::NB; n++) {
         const int64_t gn = g0 + 16 * n + j;
@@ -580,9 +603,8 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
             for (int u = 0; u < Gm::ROWS; u++) {
                 const int row = Gm::slot_row(q, u);
                 if (row < Gm::K) {
-                    const double v = shifted
-                        ? fma(-1.0, ystar[row], Y[n][u])
-                        : Y[n][u];
+                    const double v = shifted ? fma(-1.0, ys[u], Y[n][u])
+                                             : Y[n][u];
                     st<ST>(k.d_out, (uint64_t)row * k.dstride + gn, v);
                 }
             }
