@@ -549,10 +549,14 @@ int fail(mbh_ctx* ctx, int code, const char* fmt, ...)
     return code;
 }
 
+/* A failed HIP call also leaves its code as the thread's last error,
+ * which a later launch check (hipGetLastError after a launch) would
+ * report as its own: the failure paths read it off. */
 #define HIP_TRY(ctx, call)                                                    \
     do {                                                                      \
         hipError_t e_ = (call);                                               \
         if (e_ != hipSuccess) {                                               \
+            (void)hipGetLastError();                                          \
             return fail(ctx, (e_ == hipErrorOutOfMemory) ? MBH_ERR_NOMEM       \
                                                          : MBH_ERR_HIP,       \
                         "%s failed: %s", #call, hipGetErrorString(e_));       \
@@ -568,6 +572,7 @@ int dalloc(mbh_ctx* ctx, void** p, size_t bytes)
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
+        (void)hipGetLastError(); /* not to be reported by a later launch */
         return fail(ctx, (e == hipErrorOutOfMemory) ? MBH_ERR_NOMEM
                                                     : MBH_ERR_HIP,
                     "hipMalloc(%zu bytes) failed: %s", bytes,

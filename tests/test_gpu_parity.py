@@ -177,6 +177,33 @@ def test_capi_direct(gpu):
     ctx.close()
 
 
+
+def test_capi_device_allocation_failure(gpu):
+    """A record scratch beyond the device's memory (2^36 doubles,
+    512 GiB): mbh_create returns MBH_ERR_NOMEM (the reference checks every
+    device allocation the same way, common_cuda.hpp:60-66) after releasing
+    what it had allocated, and leaves no HIP error behind: the next context
+    in the same process runs bit-identical to the oracle."""
+    spec = setups.sit_2lvl(num_gridpoints=300, endtime=0.1e-12,
+                           num_records=4)
+    dev, scen = setups.build(spec)
+    prob = mb.build_problem("hip-fdtd-2lvl-reg-cayley", dev, scen)
+    big = dict(prob)
+    big["scratch_size"] = 1 << 36
+    print("case allocation failure", flush=True)
+    with pytest.raises(capi.MBHError, match="hipMalloc") as ei:
+        capi.Context([big], capi.options("fp64"))
+    assert ei.value.code == 3  # MBH_ERR_NOMEM
+    print("case run after the failure", flush=True)
+    ctx = capi.Context([prob], capi.options("fp64"))
+    ctx.run(0, int(prob["num_timesteps"]))
+    scratch = ctx.records(0)
+    ref = oracle.OracleRun(prob).run()
+    assert np.any(scratch != 0.0)
+    assert np.array_equal(scratch, ref.scratch[:len(scratch)])
+    ctx.close()
+
+
 MFMA_TOL = 1e-6  # split-FP16 generator product, short runs
 
 
