@@ -152,6 +152,26 @@ __device__ __forceinline__ double reduce_scatter4(const double* v)
     return s0 + s1;
 }
 
+/* v - (float)hi for both halves of a packed FP16 pair, each as one
+ * mixed-precision FMA (v_fma_mix_f32 reads the FP16 half directly; the
+ * difference is exact in FP32, as the separate conversion and subtraction
+ * it replaces) */
+__device__ __forceinline__ zm_f2 zm_rest2(zm_h2 hi, zm_f2 v)
+{
+    const unsigned hp = __builtin_bit_cast(unsigned, hi);
+    float r0, r1;
+    asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]"
+        : "=v"(r0)
+        : "v"(hp), "v"(v[0]));
+    asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+        : "=v"(r1)
+        : "v"(hp), "v"(v[1]));
+    zm_f2 r;
+    r[0] = r0;
+    r[1] = r1;
+    return r;
+}
+
 /* per-material constants of one lane */
 template<int N>
 struct zm_consts {
@@ -227,7 +247,7 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
         /* r = hi + lo: hi = f16(r) and lo = f16(r - hi), both rounded to
          * nearest, converted two at a time (v_cvt_pk_f16_f32); r - hi is
          * exact in FP32 and is formed from the packed half directly
-         * (mixed-precision FMA) */
+         * (zm_rest2) */
         zm_h8 bh[Gm::KC], bl[Gm::KC];
 #pragma unroll
         for (int kc = 0; kc < Gm::KC; kc++) {
@@ -242,10 +262,8 @@ __device__ __forceinline__ void zm_step(const zm_consts<N>& K_,
                         : ((Gm::CFOLD && u == Gm::ROWS) ? one_q0 : 0.0f);
                 }
                 const zm_h2 hi = __builtin_convertvector(v, zm_h2);
-                zm_f2 rest;
-                rest[0] = __builtin_fmaf((float)hi[0], -1.0f, v[0]);
-                rest[1] = __builtin_fmaf((float)hi[1], -1.0f, v[1]);
-                const zm_h2 lo = __builtin_convertvector(rest, zm_h2);
+                const zm_h2 lo =
+                    __builtin_convertvector(zm_rest2(hi, v), zm_h2);
                 bh[kc][sl] = hi[0];
                 bh[kc][sl + 1] = hi[1];
                 bl[kc][sl] = lo[0];
