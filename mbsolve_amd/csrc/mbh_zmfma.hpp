@@ -464,9 +464,23 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
             type == MBH_REC_INVERSION;
     }
 
-    for (uint32_t s = 0; s < nsteps; s++) {
+    /* One step in three parts.  The field part (E update, sources, H
+     * update and boundary condition, the field's broadcast to the MFMA
+     * rows) returns the wave-uniform sin/cos tier of the density step (0
+     * without a density); the density part runs zm_step at that tier; the
+     * record part copies the step's records.  With ZM_TIER0_LOOP (six
+     * levels) the block first runs a loop of tier-0 steps (the tier of
+     * weak fields: one code path, so one register assignment for the
+     * state, without the copies where four tier paths meet) until a step
+     * needs a wider tier, then finishes that step and the rest of the
+     * block with the tier chosen step by step.  Each step makes the same
+     * tier decision either way, so the results do not depend on the loop
+     * that ran it.  (Measured: six levels 13 % faster; three levels, where
+     * the register pressure is low, 2.5 % slower.) */
+    constexpr bool ZM_TIER0_LOOP = (N >= 6);
+    double eb[Gm::NB];
+    auto field_step = [&](uint32_t s) __attribute__((always_inline)) -> int {
         const uint64_t n = n0 + s;
-
         /* E update */
         const double h_right = from_right(h);
         double en = fma(fx, h_right - h, fma(fg, p, fa * e));
@@ -487,34 +501,40 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
         const double e_left = from_left(en);
         h = fma(fc, en - e_left, hm * h);
         e = en;
-
-        /* density update */
-        double eb[Gm::NB];
         static_assert(Gm::NB == 4, "four 16-lane rows per wavefront");
         rows_broadcast(e, eb);
-        if (q0_active) {
-            double emax = fabs(eb[0]);
-#pragma unroll
-            for (int nb = 1; nb < Gm::NB; nb++) {
-                emax = fmax(emax, fabs(eb[nb]));
-            }
-            const double xmax = KC0.thmax * emax;
-            const int tier = __all(xmax <= 0x1p-13) ? 0
-                : (__all(xmax <= 0x1p-10) ? 1
-                   : (__all(xmax <= TAYLOR_MAX) ? 2 : 3));
-            double pp[Gm::NB];
-            if (tier == 0) {
-                zm_step<N, 0>(KC0, s_frag, lane, eb, one_q0, Y, pp);
-            } else if (tier == 1) {
-                zm_step<N, 1>(KC0, s_frag, lane, eb, one_q0, Y, pp);
-            } else if (tier == 2) {
-                zm_step<N, 2>(KC0, s_frag, lane, eb, one_q0, Y, pp);
-            } else {
-                zm_step<N, 3>(KC0, s_frag, lane, eb, one_q0, Y, pp);
-            }
-            p = reduce_scatter4(pp) + KC0.b;
+        if (!q0_active) {
+            return 0;
         }
-
+        double emax = fabs(eb[0]);
+#pragma unroll
+        for (int nb = 1; nb < Gm::NB; nb++) {
+            emax = fmax(emax, fabs(eb[nb]));
+        }
+        const double xmax = KC0.thmax * emax;
+        return __all(xmax <= 0x1p-13) ? 0
+            : (__all(xmax <= 0x1p-10) ? 1
+               : (__all(xmax <= TAYLOR_MAX) ? 2 : 3));
+    };
+    auto density_step = [&](auto tier_c) __attribute__((always_inline)) {
+        double pp[Gm::NB];
+        zm_step<N, decltype(tier_c)::value>(KC0, s_frag, lane, eb, one_q0, Y,
+                                            pp);
+        p = reduce_scatter4(pp) + KC0.b;
+    };
+    auto density_any = [&](int tier) __attribute__((always_inline)) {
+        if (tier == 0) {
+            density_step(std::integral_constant<int, 0>());
+        } else if (tier == 1) {
+            density_step(std::integral_constant<int, 1>());
+        } else if (tier == 2) {
+            density_step(std::integral_constant<int, 2>());
+        } else {
+            density_step(std::integral_constant<int, 3>());
+        }
+    };
+    auto record_step = [&](uint32_t s) __attribute__((always_inline)) {
+        const uint64_t n = n0 + s;
         /* record copy */
         if ((rec_steps >> s) & 1ull) {
             /* diagonal coherence-vector components of this lane's cell:
@@ -597,6 +617,42 @@ __global__ __launch_bounds__(64) ZM_OCCUPANCY void zm_kernel(KArgs k)
                         v;
                 }
             }
+        }
+    };
+
+    if constexpr (ZM_TIER0_LOOP) {
+        uint32_t s = 0;
+        int pending = 0;
+        for (; s < nsteps; s++) {
+            const int tier = field_step(s);
+            if (tier != 0) {
+                pending = tier;
+                break;
+            }
+            if (q0_active) {
+                density_step(std::integral_constant<int, 0>());
+            }
+            record_step(s);
+        }
+        if (s < nsteps) {
+            /* step s has its field part done and needs a wider tier */
+            density_any(pending);
+            record_step(s);
+            for (s++; s < nsteps; s++) {
+                const int tier = field_step(s);
+                if (q0_active) {
+                    density_any(tier);
+                }
+                record_step(s);
+            }
+        }
+    } else {
+        for (uint32_t s = 0; s < nsteps; s++) {
+            const int tier = field_step(s);
+            if (q0_active) {
+                density_any(tier);
+            }
+            record_step(s);
         }
     }
 
