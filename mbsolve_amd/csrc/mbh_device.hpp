@@ -909,9 +909,10 @@ struct algo_zform {
          * read from LDS) the rows are accumulated side by side (k outer):
          * K independent FMA chains instead of one chain per row after
          * another, at the price of holding W in VGPRs during the product.
-         * Several cells per lane already interleave C chains, and larger K
-         * keeps one row at a time (register pressure). */
-        constexpr int RG = (K <= 8 && C == 1) ? K : 1;
+         * Larger K (the 35 components of six levels) interleaves two rows:
+         * half the chain latency without spilling to AGPRs (three rows
+         * do).  Several cells per lane already interleave C chains. */
+        constexpr int RG = C != 1 ? 1 : (K <= 8 ? K : 2);
 #pragma unroll
         for (int i0 = 0; i0 < K; i0 += RG) {
             double acc[C][RG];
