@@ -921,6 +921,7 @@ KArgs make_args(const mbh_ctx* ctx, int in_buf, uint64_t n_off,
 hipError_t launch_block(const mbh_ctx* ctx, KArgs k);
 
 constexpr uint32_t FP32_MIN_BLOCK = 8;
+constexpr double FP32_TIE = 0.015;
 
 /*
  * Autotuner: times every admissible (cells per lane, temporal block) pair
@@ -990,6 +991,7 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
      * graph size) */
     double best = 1e300, spent_ms = 0.0;
     std::pair<uint32_t, uint32_t> pick = cand[0];
+    std::vector<std::pair<double, std::pair<uint32_t, uint32_t>>> timed;
     size_t tried = 0;
     hipLaunchKernelGGL(set_u64, dim3(1), dim3(1), 0, ctx->stream,
                        ctx->n_base, (uint64_t)0);
@@ -1034,9 +1036,25 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
         spent_ms += ms0;
         tried++;
         const double per_step = (double)ms / (reps * (double)steps);
+        timed.emplace_back(per_step, cd);
         if (per_step < best) {
             best = per_step;
             pick = cd;
+        }
+    }
+    /* FP32 storage: among candidates within FP32_TIE of the fastest (the
+     * spread of the trial timings), the deepest block, whose runs round
+     * their state the fewest times */
+    if (ctx->fp32) {
+        const double limit = best * (1.0 + FP32_TIE);
+        for (const auto& tc : timed) {
+            const bool deeper = tc.second.second > pick.second;
+            const bool same_faster =
+                tc.second.second == pick.second && tc.first < best;
+            if (tc.first <= limit && (deeper || same_faster)) {
+                pick = tc.second;
+                best = tc.first;
+            }
         }
     }
     ctx->tuned_candidates = tried;
