@@ -216,6 +216,28 @@ def summarize(errs, names):
     return out
 
 
+def pick_headline(cands, results, tol, have_ref):
+    """The fastest candidate form whose error is within tol (every
+    candidate counts when none is) and whether one was (None without
+    reference records, when no error is known)."""
+    ok = [f for f in cands if "errs" not in results[f] or
+          results[f]["errs"]["all_max"] <= tol]
+    within_tol = bool(ok) if have_ref else None
+    return max(ok or cands, key=lambda f: results[f]["value"]), within_tol
+
+
+def fastest_exact64(results, fp64):
+    """(form, value) of the fastest form with FP64 arithmetic throughout
+    among the measured forms and the plain FP64 timing ("fp64"), or None."""
+    exact64 = {f: results[f]["value"] for f in results if f in EXACT64}
+    if fp64 is not None:
+        exact64["fp64"] = fp64["value"]
+    if not exact64:
+        return None
+    f = max(exact64, key=exact64.get)
+    return f, exact64[f]
+
+
 def timed_passes(b, steps, warmup, passes):
     """`warmup` untimed passes, then `passes` timed ones between device
     synchronisations (wall clock, and the summed device time of the
@@ -499,10 +521,8 @@ def main():
         else:
             b.close()
 
-    ok = [f for f in cands if "errs" not in results[f] or
-          results[f]["errs"]["all_max"] <= args.tol]
-    within_tol = bool(ok) if ref_recs is not None else None
-    head = max(ok or cands, key=lambda f: results[f]["value"])
+    head, within_tol = pick_headline(cands, results, args.tol,
+                                     ref_recs is not None)
     for f in list(benches):
         if f != head:
             benches.pop(f).close()
@@ -623,14 +643,10 @@ def main():
         out["fp64_value"] = fp64["value"]
         out["fp64_steps"] = fp64["steps"]
         out["speedup_vs_fp64"] = value / fp64["value"]
-    exact64 = {f: results[f]["value"] for f in results if f in EXACT64}
-    if fp64 is not None:
-        exact64["fp64"] = fp64["value"]
-    if exact64:
-        f64 = max(exact64, key=exact64.get)
+    ex64 = fastest_exact64(results, fp64)
+    if ex64 is not None:
         out["speedup_vs_fastest_fp64_form"] = {
-            "form": f64, "value": exact64[f64],
-            "speedup": value / exact64[f64]}
+            "form": ex64[0], "value": ex64[1], "speedup": value / ex64[1]}
     out["forms"] = {f: {"value": r["value"], "storage": STORAGE[f],
                         "max_rel_err": (r["errs"]["all_max"]
                                         if "errs" in r else None),
