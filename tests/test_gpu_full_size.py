@@ -83,10 +83,12 @@ def _sweep_subset():
 
 
 # FP32 storage rounds every run's state at each launch boundary; over the
-# 49114 steps the error grows with the number of those roundings: at the
-# 16-step blocks the autotuner picks for an eight-run batch the worst run
-# of this subset reaches 1.1e-5 (at the 32-step blocks of the 512-run
-# batch, 5.7e-6 over all 512 runs), so the FP32 form is held to 2e-5 here.
+# 49114 steps the error grows with the number of those roundings: at
+# 16-step blocks the worst run of this subset reaches 1.1e-5 (at 32-step
+# blocks of the 512-run batch, 5.7e-6 over all 512 runs).  The autotuner
+# takes the deepest block within its trials' spread for FP32 storage, but
+# an eight-run batch can still tune to 16, so the FP32 form is held to 2e-5
+# here.
 FP32_SWEEP_TOL = 2e-5
 
 
