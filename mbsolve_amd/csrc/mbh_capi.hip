@@ -1042,16 +1042,17 @@ int autotune(mbh_ctx* ctx, uint32_t req_tb, uint32_t req_tile)
             pick = cd;
         }
     }
-    /* FP32 storage: among candidates within FP32_TIE of the fastest (the
-     * spread of the trial timings), the deepest block, whose runs round
-     * their state the fewest times */
+    /* FP32 storage: among candidates of the fastest one's tile width
+     * within FP32_TIE of it (the spread of the trial timings), the deepest
+     * block, whose runs round their state the fewest times.  (A tile of
+     * four wavefronts inside that spread has run its full pass 5 % slower
+     * than the fastest width.) */
     if (ctx->fp32) {
         const double limit = best * (1.0 + FP32_TIE);
+        const uint32_t width = pick.first;
         for (const auto& tc : timed) {
-            const bool deeper = tc.second.second > pick.second;
-            const bool same_faster =
-                tc.second.second == pick.second && tc.first < best;
-            if (tc.first <= limit && (deeper || same_faster)) {
+            if (tc.second.first == width && tc.first <= limit &&
+                tc.second.second > pick.second) {
                 pick = tc.second;
                 best = tc.first;
             }
